@@ -1,0 +1,634 @@
+// C ABI implementation (include/cp3_hip.h): the single-grid atmospheric and
+// beam propagators.  Host logic mirrors AtmosCpuPropagator / BeamCpuPropagator
+// (validation, call order, Earth model handling) and AtmosCudaPropagatorSingle
+// / BeamCudaPropagatorSingle (device residency, stream, lazy result copy);
+// all state is per object.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cp3_hip.h"
+#include "cp3_capi_util.hpp"
+#include "cp3_host.hpp"
+#include "cp3_launch.hpp"
+
+using cp3::capi::DevBuf;
+using cp3::capi::DeviceScope;
+using cp3::capi::Error;
+using cp3::capi::guard;
+using cp3::capi::hip_check;
+using cp3::capi::PinnedBuf;
+
+namespace {
+
+constexpr int kVersion = 10000;  // 1.0.0
+
+void check_precision(int precision) {
+  if (precision != CP3_FP64 && precision != CP3_FP32 && precision != CP3_MIXED)
+    throw Error(CP3_ERR_INVALID, "unknown precision " + std::to_string(precision));
+}
+
+void check_nutype(int nutype) {
+  if (nutype != 0 && nutype != 1)
+    throw Error(CP3_ERR_INVALID, "neutrino type must be 0 (neutrino) or 1 (antineutrino)");
+}
+
+void check_channel(int t) {
+  if (t < 0 || t > 8) throw Error(CP3_ERR_RANGE, "probability type must be in 0..8");
+}
+
+void open_device(int device) {
+  int n = 0;
+  hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+  if (n == 0) throw Error(CP3_ERR_DEVICE, "No GPU found");
+  if (device < 0 || device >= n)
+    throw Error(CP3_ERR_INVALID,
+                "The requested GPU Id " + std::to_string(device) + " is not available.");
+}
+
+// Oscillation parameters as the reference setters store them.
+struct OscArgs {
+  double th12 = 0, th13 = 0, th23 = 0, dcp = 0;
+  int knu = 1;
+  double dm12 = 0, dm23 = 0;
+  cp3::SetConst make(int nutype) const {
+    return cp3::make_set(th12, th13, th23, dcp, knu, dm12, dm23, nutype);
+  }
+};
+
+// Largest cosine block per precision that keeps the state in registers at a
+// useful occupancy; the work-unit count decides below that.
+int auto_nc(int precision, const cp3::GridDev& g) {
+  const int cap = precision == CP3_FP64 ? 2 : (precision == CP3_FP32 ? 8 : 4);
+  for (int nc = cap; nc > 1; nc /= 2)
+    if (cp3::atmos_units(g, nc) >= 4096) return nc;
+  return 1;
+}
+
+}  // namespace
+
+struct cp3_atmos {
+  int dev = 0, nC = 0, nE = 0, precision = CP3_FP64, nc_force = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool launched = false;
+  std::vector<double> energies, cosines;
+  bool cos_set = false, height_set = false;
+  double h_cm = 0.0;
+  OscArgs osc;
+  cp3::EarthModel earth;
+  int n_layers = 0;
+  DevBuf d_ev, d_perm, d_M, d_Lvac, d_seg, d_rye, d_out;
+  PinnedBuf h_out;
+  int S = 0;
+  bool table_dirty = true, energies_dirty = true, results_on_host = false;
+
+  ~cp3_atmos() {
+    DeviceScope ds(dev);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  size_t out_bytes() const {
+    return (size_t)9 * (size_t)nC * (size_t)nE * (size_t)cp3::out_elem_bytes(precision);
+  }
+
+  void upload_tables() {
+    if (energies_dirty) {
+      d_ev.upload(energies.data(), energies.size() * sizeof(double), dev);
+      energies_dirty = false;
+    }
+    if (table_dirty) {
+      cp3::TrajTable tt;
+      cp3::build_traj_table(tt, earth, cosines.data(), nC, h_cm);
+      S = tt.S;
+      d_perm.upload(tt.perm.data(), tt.perm.size() * sizeof(int32_t), dev);
+      d_M.upload(tt.M.data(), tt.M.size() * sizeof(int32_t), dev);
+      d_Lvac.upload(tt.Lvac.data(), tt.Lvac.size() * sizeof(double), dev);
+      d_seg.upload(tt.seg.data(), tt.seg.size() * sizeof(double), dev);
+      d_rye.upload(tt.rye.data(), tt.rye.size() * sizeof(double), dev);
+      table_dirty = false;
+    }
+  }
+
+  cp3::GridDev grid() const {
+    cp3::GridDev g{};
+    g.ev = (const double*)d_ev.p;
+    g.perm = (const int32_t*)d_perm.p;
+    g.M = (const int32_t*)d_M.p;
+    g.Lvac = (const double*)d_Lvac.p;
+    g.seg = (const double*)d_seg.p;
+    g.rye = (const double*)d_rye.p;
+    g.nC = nC;
+    g.nE = nE;
+    g.S = S;
+    return g;
+  }
+
+  void launch(int nutype) {
+    if (!height_set)
+      throw Error(CP3_ERR_STATE,
+                  "CudaPropagatorSingle::calculateProbabilities. production height was not set");
+    DeviceScope ds(dev);
+    upload_tables();
+    cp3::AtmosJob job{};
+    job.sc = osc.make(nutype);
+    job.g = grid();
+    job.out = d_out.p;
+    job.unit_begin = 0;
+    const int nc = nc_force ? nc_force : auto_nc(precision, job.g);
+    results_on_host = false;
+    hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+    hip_check(cp3::launch_atmos_single(precision, nc, job, stream), "atmospheric kernel launch");
+    hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+    launched = true;
+  }
+
+  void wait() {
+    DeviceScope ds(dev);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  void fetch() {
+    if (results_on_host) return;
+    DeviceScope ds(dev);
+    hip_check(hipMemcpyAsync(h_out.p, d_out.p, out_bytes(), hipMemcpyDeviceToHost, stream),
+              "result copy");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    results_on_host = true;
+  }
+
+  double value(size_t idx) const {
+    if (precision == CP3_FP64) return ((const double*)h_out.p)[idx];
+    return (double)((const float*)h_out.p)[idx];
+  }
+};
+
+struct cp3_beam {
+  int dev = 0, nE = 0, precision = CP3_FP64;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool launched = false;
+  std::vector<double> energies;
+  double rho = 0.0, L = 0.0;
+  OscArgs osc;
+  DevBuf d_ev, d_out;
+  PinnedBuf h_out;
+  bool energies_dirty = true, results_on_host = false;
+
+  ~cp3_beam() {
+    DeviceScope ds(dev);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  size_t out_bytes() const {
+    return (size_t)9 * (size_t)nE * (size_t)cp3::out_elem_bytes(precision);
+  }
+  void launch(int nutype) {
+    DeviceScope ds(dev);
+    if (energies_dirty) {
+      d_ev.upload(energies.data(), energies.size() * sizeof(double), dev);
+      energies_dirty = false;
+    }
+    cp3::BeamJob job{};
+    job.sc = osc.make(nutype);
+    job.ev = (const double*)d_ev.p;
+    job.nE = nE;
+    job.rye = rho * cp3::kBeamYe;
+    job.L = L;
+    job.out = d_out.p;
+    results_on_host = false;
+    hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+    hip_check(cp3::launch_beam_single(precision, job, stream), "beam kernel launch");
+    hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+    launched = true;
+  }
+  void wait() {
+    DeviceScope ds(dev);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+  void fetch() {
+    if (results_on_host) return;
+    DeviceScope ds(dev);
+    hip_check(hipMemcpyAsync(h_out.p, d_out.p, out_bytes(), hipMemcpyDeviceToHost, stream),
+              "result copy");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    results_on_host = true;
+  }
+  double value(size_t idx) const {
+    if (precision == CP3_FP64) return ((const double*)h_out.p)[idx];
+    return (double)((const float*)h_out.p)[idx];
+  }
+};
+
+namespace {
+
+template <class T>
+T* need(T* p) {
+  if (!p) throw Error(CP3_ERR_INVALID, "null propagator handle");
+  return p;
+}
+
+void need_ptr(const void* p, const char* what) {
+  if (!p) throw Error(CP3_ERR_INVALID, std::string("null pointer: ") + what);
+}
+
+float event_ms(hipEvent_t a, hipEvent_t b, int dev) {
+  DeviceScope ds(dev);
+  hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+  return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cp3_last_error(void) { return cp3::capi::last_error().c_str(); }
+
+int cp3_version(void) { return kVersion; }
+
+int cp3_device_count(int* n) {
+  return guard([&] {
+    need_ptr(n, "n");
+    int c = 0;
+    hip_check(hipGetDeviceCount(&c), "hipGetDeviceCount");
+    *n = c;
+  });
+}
+
+// ------------------------------------------------------------- atmospheric
+
+int cp3_atmos_create(int device, int n_cosines, int n_energies, int precision,
+                     cp3_atmos** out) {
+  return guard([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    if (n_cosines <= 0 || n_energies <= 0)
+      throw Error(CP3_ERR_INVALID, "number of cosines and energies must be positive");
+    check_precision(precision);
+    open_device(device);
+    std::unique_ptr<cp3_atmos> p(new cp3_atmos());
+    p->dev = device;
+    p->nC = n_cosines;
+    p->nE = n_energies;
+    p->precision = precision;
+    p->energies.assign((size_t)n_energies, 0.0);
+    p->cosines.assign((size_t)n_cosines, 0.0);
+    DeviceScope ds(device);
+    hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&p->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&p->ev1), "hipEventCreate");
+    p->d_out.alloc(p->out_bytes(), device);
+    p->h_out.alloc(p->out_bytes());
+    *out = p.release();
+  });
+}
+
+int cp3_atmos_destroy(cp3_atmos* p) {
+  return guard([&] { delete p; });
+}
+
+int cp3_atmos_set_energy_list(cp3_atmos* p, const double* e, int n) {
+  return guard([&] {
+    need(p);
+    if (n != p->nE)
+      throw Error(CP3_ERR_INVALID,
+                  "Propagator::setEnergyList. Propagator was not created for this number of energy nodes");
+    need_ptr(e, "energies");
+    p->energies.assign(e, e + n);
+    p->energies_dirty = true;
+  });
+}
+
+int cp3_atmos_set_cosine_list(cp3_atmos* p, const double* c, int n) {
+  return guard([&] {
+    need(p);
+    if (n != p->nC)
+      throw Error(CP3_ERR_INVALID,
+                  "Propagator::setCosineList. Propagator was not created for this number of cosine nodes");
+    need_ptr(c, "cosines");
+    p->cosines.assign(c, c + n);
+    p->cos_set = true;
+    p->table_dirty = true;
+  });
+}
+
+int cp3_atmos_set_mns_matrix(cp3_atmos* p, double t12, double t13, double t23,
+                             double dcp, int knu) {
+  return guard([&] {
+    need(p);
+    p->osc.th12 = t12;
+    p->osc.th13 = t13;
+    p->osc.th23 = t23;
+    p->osc.dcp = dcp;
+    p->osc.knu = knu;
+  });
+}
+
+int cp3_atmos_set_neutrino_masses(cp3_atmos* p, double dm12, double dm23) {
+  return guard([&] {
+    need(p);
+    p->osc.dm12 = dm12;
+    p->osc.dm23 = dm23;
+  });
+}
+
+int cp3_atmos_set_density(cp3_atmos* p, const double* r, const double* rho,
+                          const double* ye, int n) {
+  return guard([&] {
+    need(p);
+    if (n < 0) throw Error(CP3_ERR_INVALID, "setDensity : negative size");
+    if (n > 0) {
+      need_ptr(r, "radii");
+      need_ptr(rho, "rhos");
+      need_ptr(ye, "yps");
+    }
+    std::vector<double> vr, vd, vy;
+    if (n > 0) {
+      vr.assign(r, r + n);
+      vd.assign(rho, rho + n);
+      vy.assign(ye, ye + n);
+    }
+    cp3::earth_set(p->earth, vr, vd, vy);
+    p->n_layers = (int)p->earth.radii.size();
+    p->table_dirty = true;
+  });
+}
+
+int cp3_atmos_set_density_from_file(cp3_atmos* p, const char* path) {
+  return guard([&] {
+    need(p);
+    need_ptr(path, "path");
+    std::vector<double> r, d, y;
+    cp3::earth_read_file(path, r, d, y);
+    cp3::earth_set(p->earth, r, d, y);
+    p->n_layers = (int)p->earth.radii.size();
+    p->table_dirty = true;
+  });
+}
+
+int cp3_atmos_set_chemical_composition(cp3_atmos* p, const double* ye, int n) {
+  return guard([&] {
+    need(p);
+    if (n < 0 || (size_t)n != p->earth.yps.size())
+      throw Error(CP3_ERR_INVALID,
+                  "CpuPropagator::setChemicalComposition. Size of input list not equal to expectation.");
+    if (n > 0) need_ptr(ye, "yps");
+    cp3::earth_set_yps(p->earth, std::vector<double>(ye, ye + n));
+    p->table_dirty = true;
+  });
+}
+
+int cp3_atmos_set_production_height(cp3_atmos* p, double km) {
+  return guard([&] {
+    need(p);
+    if (!p->cos_set)
+      throw Error(CP3_ERR_STATE, "must set cosine list before production height");
+    p->h_cm = km * 100000.0;
+    p->height_set = true;
+    p->table_dirty = true;
+  });
+}
+
+int cp3_atmos_get_nlayer_boundaries(cp3_atmos* p, int* n) {
+  return guard([&] {
+    need(p);
+    need_ptr(n, "n");
+    *n = p->n_layers;
+  });
+}
+
+int cp3_atmos_calculate(cp3_atmos* p, int nutype) {
+  return guard([&] {
+    need(p);
+    check_nutype(nutype);
+    p->launch(nutype);
+    p->wait();
+  });
+}
+
+int cp3_atmos_calculate_async(cp3_atmos* p, int nutype) {
+  return guard([&] {
+    need(p);
+    check_nutype(nutype);
+    p->launch(nutype);
+  });
+}
+
+int cp3_atmos_wait(cp3_atmos* p) {
+  return guard([&] { need(p)->wait(); });
+}
+
+int cp3_atmos_get_probability(cp3_atmos* p, int ic, int ie, int t, double* out) {
+  return guard([&] {
+    need(p);
+    need_ptr(out, "out");
+    if (ic < 0 || ie < 0 || ic >= p->nC || ie >= p->nE)
+      throw Error(CP3_ERR_RANGE, "CudaPropagatorSingle::getProbability. Invalid indices");
+    check_channel(t);
+    if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
+    p->fetch();
+    *out = p->value(((size_t)t * p->nC + (size_t)ic) * p->nE + (size_t)ie);
+  });
+}
+
+int cp3_atmos_get_probability_arr(cp3_atmos* p, double* out, int t) {
+  return guard([&] {
+    need(p);
+    need_ptr(out, "out");
+    check_channel(t);
+    if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
+    p->fetch();
+    const size_t base = (size_t)t * p->nC * p->nE;
+    size_t k = 0;
+    for (int ie = 0; ie < p->nE; ++ie)
+      for (int ic = 0; ic < p->nC; ++ic)
+        out[k++] = p->value(base + (size_t)ic * p->nE + (size_t)ie);
+  });
+}
+
+int cp3_atmos_result_device(cp3_atmos* p, const void** dptr, int* elem) {
+  return guard([&] {
+    need(p);
+    need_ptr(dptr, "dptr");
+    *dptr = p->d_out.p;
+    if (elem) *elem = cp3::out_elem_bytes(p->precision);
+  });
+}
+
+int cp3_atmos_get_stream(cp3_atmos* p, void** s) {
+  return guard([&] {
+    need(p);
+    need_ptr(s, "stream");
+    *s = (void*)p->stream;
+  });
+}
+
+int cp3_atmos_last_kernel_ms(cp3_atmos* p, float* ms) {
+  return guard([&] {
+    need(p);
+    need_ptr(ms, "ms");
+    if (!p->launched) throw Error(CP3_ERR_STATE, "no kernel was launched");
+    *ms = event_ms(p->ev0, p->ev1, p->dev);
+  });
+}
+
+int cp3_atmos_set_cosine_block(cp3_atmos* p, int nc) {
+  return guard([&] {
+    need(p);
+    if (nc != 0 && !cp3::valid_nc(nc))
+      throw Error(CP3_ERR_INVALID, "cosine block must be 0, 1, 2, 4 or 8");
+    p->nc_force = nc;
+  });
+}
+
+// -------------------------------------------------------------------- beam
+
+int cp3_beam_create(int device, int n_energies, int precision, cp3_beam** out) {
+  return guard([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    if (n_energies <= 0) throw Error(CP3_ERR_INVALID, "number of energies must be positive");
+    check_precision(precision);
+    open_device(device);
+    std::unique_ptr<cp3_beam> p(new cp3_beam());
+    p->dev = device;
+    p->nE = n_energies;
+    p->precision = precision;
+    p->energies.assign((size_t)n_energies, 0.0);
+    DeviceScope ds(device);
+    hip_check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&p->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&p->ev1), "hipEventCreate");
+    p->d_out.alloc(p->out_bytes(), device);
+    p->h_out.alloc(p->out_bytes());
+    *out = p.release();
+  });
+}
+
+int cp3_beam_destroy(cp3_beam* p) {
+  return guard([&] { delete p; });
+}
+
+int cp3_beam_set_energy_list(cp3_beam* p, const double* e, int n) {
+  return guard([&] {
+    need(p);
+    if (n != p->nE)
+      throw Error(CP3_ERR_INVALID,
+                  "Propagator::setEnergyList. Propagator was not created for this number of energy nodes");
+    need_ptr(e, "energies");
+    p->energies.assign(e, e + n);
+    p->energies_dirty = true;
+  });
+}
+
+int cp3_beam_set_mns_matrix(cp3_beam* p, double t12, double t13, double t23,
+                            double dcp, int knu) {
+  return guard([&] {
+    need(p);
+    p->osc.th12 = t12;
+    p->osc.th13 = t13;
+    p->osc.th23 = t23;
+    p->osc.dcp = dcp;
+    p->osc.knu = knu;
+  });
+}
+
+int cp3_beam_set_neutrino_masses(cp3_beam* p, double dm12, double dm23) {
+  return guard([&] {
+    need(p);
+    p->osc.dm12 = dm12;
+    p->osc.dm23 = dm23;
+  });
+}
+
+int cp3_beam_set_density(cp3_beam* p, double rho) {
+  return guard([&] { need(p)->rho = rho; });
+}
+
+int cp3_beam_set_path_length(cp3_beam* p, double L) {
+  return guard([&] { need(p)->L = L; });
+}
+
+int cp3_beam_calculate(cp3_beam* p, int nutype) {
+  return guard([&] {
+    need(p);
+    check_nutype(nutype);
+    p->launch(nutype);
+    p->wait();
+  });
+}
+
+int cp3_beam_calculate_async(cp3_beam* p, int nutype) {
+  return guard([&] {
+    need(p);
+    check_nutype(nutype);
+    p->launch(nutype);
+  });
+}
+
+int cp3_beam_wait(cp3_beam* p) {
+  return guard([&] { need(p)->wait(); });
+}
+
+int cp3_beam_get_probability(cp3_beam* p, int ie, int t, double* out) {
+  return guard([&] {
+    need(p);
+    need_ptr(out, "out");
+    if (ie < 0 || ie >= p->nE)
+      throw Error(CP3_ERR_RANGE, "CudaPropagatorSingle::getProbability. Invalid indices");
+    check_channel(t);
+    if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
+    p->fetch();
+    *out = p->value((size_t)t * p->nE + (size_t)ie);
+  });
+}
+
+int cp3_beam_get_probability_arr(cp3_beam* p, double* out, int t) {
+  return guard([&] {
+    need(p);
+    need_ptr(out, "out");
+    check_channel(t);
+    if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
+    p->fetch();
+    for (int ie = 0; ie < p->nE; ++ie) out[ie] = p->value((size_t)t * p->nE + (size_t)ie);
+  });
+}
+
+int cp3_beam_result_device(cp3_beam* p, const void** dptr, int* elem) {
+  return guard([&] {
+    need(p);
+    need_ptr(dptr, "dptr");
+    *dptr = p->d_out.p;
+    if (elem) *elem = cp3::out_elem_bytes(p->precision);
+  });
+}
+
+int cp3_beam_get_stream(cp3_beam* p, void** s) {
+  return guard([&] {
+    need(p);
+    need_ptr(s, "stream");
+    *s = (void*)p->stream;
+  });
+}
+
+int cp3_beam_last_kernel_ms(cp3_beam* p, float* ms) {
+  return guard([&] {
+    need(p);
+    need_ptr(ms, "ms");
+    if (!p->launched) throw Error(CP3_ERR_STATE, "no kernel was launched");
+    *ms = event_ms(p->ev0, p->ev1, p->dev);
+  });
+}
+
+}  // extern "C"

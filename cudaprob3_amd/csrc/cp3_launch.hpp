@@ -1,0 +1,73 @@
+// Device-side job descriptors and the kernel launchers (cp3_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cp3_math.hpp"
+
+namespace cp3 {
+
+// Output precision forms.
+enum Precision : int { kFP64 = 0, kFP32 = 1, kMixed = 2 };
+
+inline int out_elem_bytes(int precision) { return precision == kFP64 ? 8 : 4; }
+
+// One resident atmospheric grid: energies, Earth model and cosine list
+// reduced to the trajectory table (cp3_host.hpp, TrajTable).
+struct GridDev {
+  const double* ev;     // [nE] GeV
+  const int32_t* perm;  // [nC] cosine indices by decreasing shell count
+  const int32_t* M;     // [nC] shells crossed
+  const double* Lvac;   // [nC] atmosphere segment, km
+  const double* seg;    // [nC * S] km per shell
+  const double* rye;    // [S] rho * Ye per shell
+  int32_t nC, nE, S, pad;
+};
+
+// One atmospheric evaluation: a parameter set and neutrino type on a grid.
+// Output layout [9][nC][nE] (channel a*3+b = P(a->b)), element type double
+// for kFP64 and float otherwise.
+struct AtmosJob {
+  SetConst sc;
+  GridDev g;
+  void* out;
+  int64_t unit_begin;  // first work unit of this job in a batch
+};
+
+// One beam evaluation: a parameter set and neutrino type on an energy list
+// through one constant-density layer.  Output layout [9][nE].
+struct BeamJob {
+  SetConst sc;
+  const double* ev;
+  double rye;  // rho * 0.5 (physics.hpp:1219)
+  double L;    // km
+  void* out;
+  int64_t unit_begin;
+  int32_t nE, pad;
+};
+
+constexpr int kWave = 64;
+
+// Work units of one job: (cosine block of nc) x (64-energy tile).
+inline int64_t atmos_units(const GridDev& g, int nc) {
+  const int64_t cb = (g.nC + nc - 1) / nc;
+  const int64_t et = (g.nE + kWave - 1) / kWave;
+  return cb * et;
+}
+inline int64_t beam_units(int nE) { return (nE + kWave - 1) / kWave; }
+
+// Supported cosine-block sizes for the atmospheric kernel.
+inline bool valid_nc(int nc) { return nc == 1 || nc == 2 || nc == 4 || nc == 8; }
+
+hipError_t launch_atmos_single(int precision, int nc, const AtmosJob& job,
+                               hipStream_t stream);
+hipError_t launch_atmos_batch(int precision, int nc, const AtmosJob* d_jobs,
+                              int njobs, int64_t nunits, int grid,
+                              hipStream_t stream);
+hipError_t launch_beam_single(int precision, const BeamJob& job,
+                              hipStream_t stream);
+hipError_t launch_beam_batch(int precision, const BeamJob* d_jobs, int njobs,
+                             int64_t nunits, int grid, hipStream_t stream);
+
+}  // namespace cp3

@@ -1,0 +1,571 @@
+// Three-flavour Barger propagation arithmetic shared by the HIP kernels and
+// the host build of the same code (libcp3model.so, used for numerics tests and
+// operation counting).  Everything here is templated on a precision policy P
+// that fixes two scalar types:
+//   P::D - eigen-solve, phase rates, phases and their reduction;
+//   P::F - eigenvectors, the per-layer amplitude updates and the output.
+//
+// Method (Barger et al., as in physics.hpp:267-629 and 789-1150):
+//   * per (energy, shell): the matter eigenvalues from the trigonometric
+//     solution of the characteristic cubic with the reference's coefficients
+//     (physics.hpp:283-323), ordered by continuity to vacuum
+//     (physics.hpp:198-252, 327-330), and the Sylvester projectors
+//     prod_{m != k}(H - l_m) / prod_{m != k}(l_k - l_m) (physics.hpp:394-427);
+//   * per (cosine, shell): the layer matrix sum_k C_k e^{-i 2.534 l_k L / E}
+//     (physics.hpp:439-450, math.hpp:57-75) and the ordered product with the
+//     mirror for the outgoing half of the path (physics.hpp:956-990);
+//   * P(a->b) = |A[b][a]|^2 (physics.hpp:1102-1144).
+//
+// Exact rewrites used to cut the arithmetic (all algebraic identities):
+//   1. With the PDG matrix U = W O G^+ (W = R23 diag(1,1,e^{i d}),
+//      O = R13 R12, G = diag(1,1,e^{i d})), the flavour Hamiltonian is
+//      W H' W^+ with H' = O diag(DM(k,0)) O^T - fac e_0 e_0^T real symmetric.
+//      Layer matrices in that frame are complex symmetric, so the mirrored
+//      product T_1 ... T_{M-1} equals (T_{M-1} ... T_1)^T and the path is
+//      A' = Y^T T_M Y T_0 with one 3x3 product chain Y instead of two.
+//   2. The projectors of the real symmetric H' are rank one, v_k v_k^T, so a
+//      layer acts as y -> y + sum_k (e^{i psi_k} - 1) v_k (v_k . y) after
+//      removing the common phase e^{i phi_ref} (a global phase of the layer,
+//      which no |A|^2 sees).  The reference index is one of the two closest
+//      eigenvalues so that an ill-conditioned eigenvector of a near-degenerate
+//      pair only ever multiplies a small e^{i psi} - 1.
+//   3. Y is unitary, so only two columns are carried; the third is
+//      det(Y) conj(y_0 x y_1), with det(Y) = e^{i sum psi} accumulated.
+//   4. cos(t0 -+ 2pi/3) = -cos(t0)/2 +- (sqrt(3)/2) sin(t0): one sincos for
+//      the three roots.
+#pragma once
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CP3_HD __host__ __device__ __forceinline__
+#else
+#define CP3_HD inline
+#endif
+
+#include <math.h>
+#include <stdint.h>
+
+namespace cp3 {
+
+// constants.hpp:28-40 and physics.hpp:444.
+constexpr double kGf2 = 1.52588e-4;
+constexpr double kLoE = 2.534;
+constexpr double kREarthKm = 6371.0;
+constexpr double kKm2Cm = 1.0e5;
+constexpr double kREarthCm = kREarthKm * kKm2Cm;
+constexpr double kBeamYe = 0.5;
+constexpr double kSqrt3Half = 0.86602540378443864676;
+constexpr double kInvTwoPi = 0.15915494309189533577;
+
+// Per (parameter set, neutrino type) constants, built on the host from the
+// reference's setter arguments.  Plain old data: copied to the device as is.
+struct SetConst {
+  double g0;          // |U_e1|^2 as physics.hpp:294 forms it
+  double q1, q2;      // 1 - |U_e2|^2, 1 - |U_e3|^2 as physics.hpp:286-291
+  double dm01, dm02;  // DM(0,1), DM(0,2) (propagator.hpp:200-208)
+  double dm00;        // DM(0,0)
+  double hv[6];       // O diag(DM(k,0)) O^T, upper triangle 00 01 02 11 12 22
+  double sgn;         // sign of the matter term: -1 neutrino, +1 antineutrino
+  double c23, s23;    // W = R23 diag(1, 1, e^{i d})
+  double cd, sd;      // cos d, sin d (d already negated for kNuType < 0)
+  int32_t order[3];   // vacuum ordering (physics.hpp:232-246)
+  int32_t pad;
+};
+
+// ---------------------------------------------------------------- policies
+
+struct PolicyF64 {
+  using D = double;
+  using F = double;
+};
+struct PolicyF32 {
+  using D = float;
+  using F = float;
+};
+struct PolicyMixed {
+  using D = double;
+  using F = float;
+};
+
+// Scalar wrappers, one overload per type, so the host build can count
+// operations by precision.
+CP3_HD double m_sqrt(double x) { return sqrt(x); }
+CP3_HD float m_sqrt(float x) { return sqrtf(x); }
+CP3_HD double m_acos(double x) { return acos(x); }
+CP3_HD float m_acos(float x) { return acosf(x); }
+CP3_HD double m_fabs(double x) { return fabs(x); }
+CP3_HD float m_fabs(float x) { return fabsf(x); }
+CP3_HD double m_rint(double x) { return rint(x); }
+CP3_HD float m_rint(float x) { return rintf(x); }
+// sin and cos of an FP64 angle, inline (no call, so nothing is spilled
+// around it): Cody-Waite reduction by pi/2 split in three parts (the first
+// two with 33 significant bits, so k*P1 and k*P2 are exact for |k| < 2^20),
+// then the Taylor series of sin and cos on [-pi/4, pi/4] through x^17 and
+// x^18 (truncation below 1e-19).  Accurate to a few ulp for |x| < 1e6 rad.
+CP3_HD void m_sincos(double x, double* s, double* c) {
+  const double kTwoOverPi = 0.6366197723675814;
+  const double P1 = 1.5707963267341256;
+  const double P2 = 6.077100506303966e-11;
+  const double P3 = 2.0222662487959506e-21;
+  const double k = rint(x * kTwoOverPi);
+  double r = fma(-k, P1, x);
+  r = fma(-k, P2, r);
+  r = fma(-k, P3, r);
+  const double z = r * r;
+  double ps = 2.8114572543455206e-15;
+  ps = fma(ps, z, -7.647163731819816e-13);
+  ps = fma(ps, z, 1.6059043836821613e-10);
+  ps = fma(ps, z, -2.505210838544172e-08);
+  ps = fma(ps, z, 2.7557319223985893e-06);
+  ps = fma(ps, z, -0.0001984126984126984);
+  ps = fma(ps, z, 0.008333333333333333);
+  ps = fma(ps, z, -0.16666666666666666);
+  const double sr = fma(r * z, ps, r);
+  double pc = -1.5619206968586225e-16;
+  pc = fma(pc, z, 4.779477332387385e-14);
+  pc = fma(pc, z, -1.1470745597729725e-11);
+  pc = fma(pc, z, 2.08767569878681e-09);
+  pc = fma(pc, z, -2.755731922398589e-07);
+  pc = fma(pc, z, 2.48015873015873e-05);
+  pc = fma(pc, z, -0.001388888888888889);
+  pc = fma(pc, z, 0.041666666666666664);
+  pc = fma(pc, z, -0.5);
+  const double cr = fma(z, pc, 1.0);
+  const int q = (int)(int64_t)k & 3;
+  const double so = (q & 1) ? cr : sr;
+  const double co = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -so : so;
+  *c = ((q + 1) & 2) ? -co : co;
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+// FP32: reduce in revolutions and use the hardware v_sin_f32 / v_cos_f32.
+CP3_HD void m_sincos(float x, float* s, float* c) {
+  float f = x * 0.15915494309189535f;
+  f = f - rintf(f);
+  *s = __builtin_amdgcn_sinf(f);
+  *c = __builtin_amdgcn_cosf(f);
+}
+#else
+CP3_HD void m_sincos(float x, float* s, float* c) {
+  *s = sinf(x);
+  *c = cosf(x);
+}
+#endif
+
+// sin and cos of 2*pi*x for |x| <= 1/2 revolution, FP32.  On the device this
+// is the hardware v_sin_f32 / v_cos_f32 (argument in revolutions).
+template <class T>
+CP3_HD void m_sincos_rev(T x, T* s, T* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  *s = __builtin_amdgcn_sinf((float)x);
+  *c = __builtin_amdgcn_cosf((float)x);
+#else
+  const double a = 6.283185307179586 * (double)x;
+  *s = (T)sin(a);
+  *c = (T)cos(a);
+#endif
+}
+
+// e^{i phase} - 1 for a layer, and the phase itself added to `acc`.
+//   FP64 / FP32: phase = om * L in radians.
+//   Mixed: om is in revolutions per km; phase reduced to [-1/2, 1/2] in FP64,
+//   then evaluated in FP32.
+template <class P>
+struct Phase;
+
+template <>
+struct Phase<PolicyF64> {
+  static constexpr double kOmScale = 1.0;
+  CP3_HD static void eval(double om, double L, double& acc, double& wr,
+                          double& wi) {
+    const double ph = om * L;
+    acc += ph;
+    double s, c;
+    m_sincos(ph, &s, &c);
+    wr = c - 1.0;
+    wi = s;
+  }
+  CP3_HD static void unit(double acc, double& cr, double& ci) {
+    m_sincos(acc, &ci, &cr);
+  }
+};
+
+template <>
+struct Phase<PolicyF32> {
+  static constexpr double kOmScale = 1.0;
+  CP3_HD static void eval(float om, float L, float& acc, float& wr,
+                          float& wi) {
+    const float ph = om * L;
+    acc += ph;
+    float s, c;
+    m_sincos(ph, &s, &c);
+    wr = c - 1.0f;
+    wi = s;
+  }
+  CP3_HD static void unit(float acc, float& cr, float& ci) {
+    m_sincos(acc, &ci, &cr);
+  }
+};
+
+template <>
+struct Phase<PolicyMixed> {
+  static constexpr double kOmScale = kInvTwoPi;
+  CP3_HD static void eval(double om, double L, double& acc, float& wr,
+                          float& wi) {
+    const double rev = om * L;
+    const double red = rev - m_rint(rev);
+    acc += red;
+    float s, c;
+    m_sincos_rev((float)red, &s, &c);
+    wr = c - 1.0f;
+    wi = s;
+  }
+  CP3_HD static void unit(double acc, float& cr, float& ci) {
+    const double red = acc - m_rint(acc);
+    m_sincos_rev((float)red, &ci, &cr);
+  }
+};
+
+// ------------------------------------------------------- per (E, shell)
+
+// Eigen-solution of one homogeneous slab: two unit eigenvectors and their
+// phase rates relative to the reference eigenvalue (radians per km, or
+// revolutions per km for the mixed policy).
+template <class P>
+struct Slab {
+  typename P::F va[3], vb[3];
+  typename P::D oma, omb;
+};
+
+template <class D>
+CP3_HD D sel3(int i, D a0, D a1, D a2) {
+  return i == 0 ? a0 : (i == 1 ? a1 : a2);
+}
+
+// Unit eigenvector of the real symmetric matrix H (upper triangle) for the
+// eigenvalue lk, from the Sylvester numerator (H - lp)(H - lq), whose columns
+// are all multiples of v_k (physics.hpp:394-427 builds the same product).
+template <class D, class F>
+CP3_HD void eigvec(const D H[6], D lk, D lp, D lq, F v[3]) {
+  const D a00 = H[0] - lp, a11 = H[3] - lp, a22 = H[5] - lp;
+  const D b00 = H[0] - lq, b11 = H[3] - lq, b22 = H[5] - lq;
+  const D h01 = H[1], h02 = H[2], h12 = H[4];
+  // P = A B with A = H - lp, B = H - lq (both symmetric, commuting).
+  const D p00 = a00 * b00 + h01 * h01 + h02 * h02;
+  const D p11 = h01 * h01 + a11 * b11 + h12 * h12;
+  const D p22 = h02 * h02 + h12 * h12 + a22 * b22;
+  const D dk = (lk - lp) * (lk - lq);
+  // P_jj = dk v_j^2: take the column with the largest |v_j|.
+  const D s0 = p00 * dk, s1 = p11 * dk, s2 = p22 * dk;
+  D c0, c1, c2, pjj;
+  if (s0 >= s1 && s0 >= s2) {
+    c0 = p00;
+    c1 = h01 * b00 + a11 * h01 + h12 * h02;
+    c2 = h02 * b00 + h12 * h01 + a22 * h02;
+    pjj = s0;
+  } else if (s1 >= s2) {
+    c0 = a00 * h01 + h01 * b11 + h02 * h12;
+    c1 = p11;
+    c2 = h02 * h01 + h12 * b11 + a22 * h12;
+    pjj = s1;
+  } else {
+    c0 = a00 * h02 + h01 * h12 + h02 * b22;
+    c1 = h01 * h02 + a11 * h12 + h12 * b22;
+    c2 = p22;
+    pjj = s2;
+  }
+  const D inv = D(1) / m_sqrt(pjj);
+  v[0] = (F)(c0 * inv);
+  v[1] = (F)(c1 * inv);
+  v[2] = (F)(c2 * inv);
+}
+
+// Matter eigenvalues in the reference's order: the trigonometric roots of
+// the cubic with physics.hpp:283-323's coefficients, reordered by ORDER.
+template <class D>
+CP3_HD void matter_eigs(const SetConst& sc, D E, D rhoYe, D lam[3]) {
+  const D fac = D(sc.sgn) * ((D(kGf2) * E) * rhoYe);
+  const D d01 = D(sc.dm01), d02 = D(sc.dm02);
+  const D alpha = (fac + d01) + d02;
+  const D beta = d01 * d02 + fac * (d01 * D(sc.q1) + d02 * D(sc.q2));
+  const D gamma = ((fac * d01) * d02) * D(sc.g0);
+  D tmp = alpha * alpha - D(3) * beta;
+  tmp = tmp < D(0) ? D(0) : tmp;
+  D arg = (D(2) * alpha * alpha * alpha - D(9) * alpha * beta +
+           D(27) * gamma) /
+          (D(2) * m_sqrt(tmp * tmp * tmp));
+  if (m_fabs(arg) > D(1)) arg = arg > D(0) ? D(1) : D(-1);
+  const D th0 = m_acos(arg) / D(3);
+  D sn, cs;
+  m_sincos(th0, &sn, &cs);
+  const D amp = D(-2.0 / 3.0) * m_sqrt(tmp);
+  const D shift = D(sc.dm00) - alpha / D(3);
+  const D hs = D(-0.5) * cs, ks = D(kSqrt3Half) * sn;
+  const D r0 = amp * cs + shift;
+  const D r1 = amp * (hs + ks) + shift;
+  const D r2 = amp * (hs - ks) + shift;
+  lam[0] = sel3(sc.order[0], r0, r1, r2);
+  lam[1] = sel3(sc.order[1], r0, r1, r2);
+  lam[2] = sel3(sc.order[2], r0, r1, r2);
+}
+
+// Solves one slab of electron density rhoYe at energy E (invE = 1/E).
+template <class P>
+CP3_HD void solve_slab(const SetConst& sc, typename P::D E,
+                       typename P::D invE, typename P::D rhoYe, Slab<P>& s) {
+  using D = typename P::D;
+  D lam[3];
+  matter_eigs<D>(sc, E, rhoYe, lam);
+  const D fac = D(sc.sgn) * ((D(kGf2) * E) * rhoYe);
+  const D H[6] = {D(sc.hv[0]) - fac, D(sc.hv[1]), D(sc.hv[2]),
+                  D(sc.hv[3]),       D(sc.hv[4]), D(sc.hv[5])};
+  // Reference index: one end of the closest pair; `near` is the other end,
+  // `far` the remaining eigenvalue.
+  const D g01 = m_fabs(lam[0] - lam[1]);
+  const D g12 = m_fabs(lam[1] - lam[2]);
+  const D g02 = m_fabs(lam[0] - lam[2]);
+  D lr, ln, lf;
+  if (g01 <= g12 && g01 <= g02) {
+    lr = lam[0]; ln = lam[1]; lf = lam[2];
+  } else if (g12 <= g02) {
+    lr = lam[2]; ln = lam[1]; lf = lam[0];
+  } else {
+    lr = lam[2]; ln = lam[0]; lf = lam[1];
+  }
+  eigvec<D>(H, lf, lr, ln, s.va);
+  eigvec<D>(H, ln, lr, lf, s.vb);
+  const D k = D(-kLoE * Phase<P>::kOmScale) * invE;
+  s.oma = k * (lf - lr);
+  s.omb = k * (ln - lr);
+}
+
+// ---------------------------------------------------- complex helpers
+
+template <class F>
+struct C3 {  // complex 3-vector
+  F re[3], im[3];
+};
+
+// y -> y + w v (v . y) with w complex and v real.
+template <class F>
+CP3_HD void rank1(const F v[3], F wr, F wi, C3<F>& y) {
+  const F tr = v[0] * y.re[0] + v[1] * y.re[1] + v[2] * y.re[2];
+  const F ti = v[0] * y.im[0] + v[1] * y.im[1] + v[2] * y.im[2];
+  const F ur = wr * tr - wi * ti;
+  const F ui = wr * ti + wi * tr;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    y.re[m] += v[m] * ur;
+    y.im[m] += v[m] * ui;
+  }
+}
+
+// ------------------------------------------------ per (cosine, shell)
+
+// Per-trajectory state.  Before the innermost shell: the first two columns
+// of Y (y0, y1) and acc = sum of the layer phases (det Y = e^{i acc}).  After
+// it: Z = Y^T T_M Y (complex symmetric) packed as y0 = (Z00, Z01, Z02),
+// y1 = (Z11, Z12, Z22).
+template <class P>
+struct Traj {
+  C3<typename P::F> y0, y1;
+  typename P::D acc;
+};
+
+template <class P>
+CP3_HD void traj_init(Traj<P>& t, bool no_shells) {
+  using F = typename P::F;
+  // Y = I (y0 = e0, y1 = e1) or, without shells, Z = I packed.  Written as
+  // value selects: conditional stores to different fields would be merged
+  // into a store through a selected address, which keeps the state out of
+  // registers.
+  const F one = F(1), zero = F(0);
+  t.y0.re[0] = one;
+  t.y0.re[1] = zero;
+  t.y0.re[2] = zero;
+  t.y1.re[0] = no_shells ? one : zero;
+  t.y1.re[1] = no_shells ? zero : one;
+  t.y1.re[2] = no_shells ? one : zero;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    t.y0.im[m] = zero;
+    t.y1.im[m] = zero;
+  }
+  t.acc = typename P::D(0);
+}
+
+// A middle shell crossed twice (physics.hpp:971-978): Y -> T Y.
+template <class P>
+CP3_HD void traj_middle(Traj<P>& t, const Slab<P>& s, typename P::D L) {
+  using F = typename P::F;
+  F wr, wi;
+  Phase<P>::eval(s.oma, L, t.acc, wr, wi);
+  rank1(s.va, wr, wi, t.y0);
+  rank1(s.va, wr, wi, t.y1);
+  Phase<P>::eval(s.omb, L, t.acc, wr, wi);
+  rank1(s.vb, wr, wi, t.y0);
+  rank1(s.vb, wr, wi, t.y1);
+}
+
+// Bilinear (unconjugated) dot products.
+template <class F>
+CP3_HD void dot_cc(const C3<F>& p, const C3<F>& q, F& r, F& i) {
+  r = p.re[0] * q.re[0] - p.im[0] * q.im[0];
+  i = p.re[0] * q.im[0] + p.im[0] * q.re[0];
+  r += p.re[1] * q.re[1] - p.im[1] * q.im[1];
+  i += p.re[1] * q.im[1] + p.im[1] * q.re[1];
+  r += p.re[2] * q.re[2] - p.im[2] * q.im[2];
+  i += p.re[2] * q.im[2] + p.im[2] * q.re[2];
+}
+template <class F>
+CP3_HD void dot_cr(const C3<F>& p, const F v[3], F& r, F& i) {
+  r = p.re[0] * v[0] + p.re[1] * v[1] + p.re[2] * v[2];
+  i = p.im[0] * v[0] + p.im[1] * v[1] + p.im[2] * v[2];
+}
+
+// The innermost shell, crossed once along its full chord
+// (physics.hpp:979-983, 785): Z = Y^T T_M Y.
+template <class P>
+CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
+  using F = typename P::F;
+  using D = typename P::D;
+  // third column y2 = det(Y) conj(y0 x y1)
+  F er, ei;
+  Phase<P>::unit(t.acc, er, ei);
+  C3<F> y2;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int p = (m + 1) % 3, q = (m + 2) % 3;
+    const C3<F>& a = t.y0;
+    const C3<F>& b = t.y1;
+    const F xr = a.re[p] * b.re[q] - a.im[p] * b.im[q] -
+                 (a.re[q] * b.re[p] - a.im[q] * b.im[p]);
+    const F xi = a.re[p] * b.im[q] + a.im[p] * b.re[q] -
+                 (a.re[q] * b.im[p] + a.im[q] * b.re[p]);
+    // e * conj(x)
+    y2.re[m] = er * xr + ei * xi;
+    y2.im[m] = ei * xr - er * xi;
+  }
+  // Z = Y^T Y + sum_k w_k u_k u_k^T with u_k = Y^T v_k; packed upper
+  // triangle 00 01 02 11 12 22.
+  F zr[6], zi[6];
+  dot_cc(t.y0, t.y0, zr[0], zi[0]);
+  dot_cc(t.y0, t.y1, zr[1], zi[1]);
+  dot_cc(t.y0, y2, zr[2], zi[2]);
+  dot_cc(t.y1, t.y1, zr[3], zi[3]);
+  dot_cc(t.y1, y2, zr[4], zi[4]);
+  dot_cc(y2, y2, zr[5], zi[5]);
+  D dummy = D(0);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const F* v = k == 0 ? s.va : s.vb;
+    F wr, wi;
+    Phase<P>::eval(k == 0 ? s.oma : s.omb, L, dummy, wr, wi);
+    F ur[3], ui[3], sr[3], si[3];
+    dot_cr(t.y0, v, ur[0], ui[0]);
+    dot_cr(t.y1, v, ur[1], ui[1]);
+    dot_cr(y2, v, ur[2], ui[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      sr[i] = wr * ur[i] - wi * ui[i];
+      si[i] = wr * ui[i] + wi * ur[i];
+    }
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+      const int i = n < 3 ? 0 : (n < 5 ? 1 : 2);
+      const int j = n < 3 ? n : (n < 5 ? n - 2 : 2);
+      zr[n] += sr[i] * ur[j] - si[i] * ui[j];
+      zi[n] += sr[i] * ui[j] + si[i] * ur[j];
+    }
+  }
+  t.y0.re[0] = zr[0]; t.y0.im[0] = zi[0];
+  t.y0.re[1] = zr[1]; t.y0.im[1] = zi[1];
+  t.y0.re[2] = zr[2]; t.y0.im[2] = zi[2];
+  t.y1.re[0] = zr[3]; t.y1.im[0] = zi[3];
+  t.y1.re[1] = zr[4]; t.y1.im[1] = zi[4];
+  t.y1.re[2] = zr[5]; t.y1.im[2] = zi[5];
+}
+
+// Applies the atmosphere (vacuum) segment, rotates back to the flavour basis
+// and forms the nine probabilities prob[a*3+b] = P(a->b) = |A[b][a]|^2.
+template <class P>
+CP3_HD void traj_finish(const Traj<P>& t, const Slab<P>& vac,
+                        typename P::D Lvac, const SetConst& sc,
+                        typename P::F prob[9]) {
+  using F = typename P::F;
+  using D = typename P::D;
+  // A = Z T_0 = Z + sum_k w_k (Z v_k) v_k^T
+  F ar[3][3], ai[3][3];
+  const F zr[3][3] = {{t.y0.re[0], t.y0.re[1], t.y0.re[2]},
+                      {t.y0.re[1], t.y1.re[0], t.y1.re[1]},
+                      {t.y0.re[2], t.y1.re[1], t.y1.re[2]}};
+  const F zi[3][3] = {{t.y0.im[0], t.y0.im[1], t.y0.im[2]},
+                      {t.y0.im[1], t.y1.im[0], t.y1.im[1]},
+                      {t.y0.im[2], t.y1.im[1], t.y1.im[2]}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      ar[i][j] = zr[i][j];
+      ai[i][j] = zi[i][j];
+    }
+  D dummy = D(0);
+  for (int k = 0; k < 2; ++k) {
+    const F* v = k == 0 ? vac.va : vac.vb;
+    F wr, wi;
+    Phase<P>::eval(k == 0 ? vac.oma : vac.omb, Lvac, dummy, wr, wi);
+    for (int i = 0; i < 3; ++i) {
+      const F tr = zr[i][0] * v[0] + zr[i][1] * v[1] + zr[i][2] * v[2];
+      const F ti = zi[i][0] * v[0] + zi[i][1] * v[1] + zi[i][2] * v[2];
+      const F sr = wr * tr - wi * ti;
+      const F si = wr * ti + wi * tr;
+      for (int j = 0; j < 3; ++j) {
+        ar[i][j] += sr * v[j];
+        ai[i][j] += si * v[j];
+      }
+    }
+  }
+  // G A G^+ with G = diag(1, 1, e^{i d})
+  const F cd = F(sc.cd), sd = F(sc.sd);
+  for (int j = 0; j < 2; ++j) {
+    const F r = ar[2][j], i = ai[2][j];
+    ar[2][j] = cd * r - sd * i;
+    ai[2][j] = cd * i + sd * r;
+    const F r2 = ar[j][2], i2 = ai[j][2];
+    ar[j][2] = cd * r2 + sd * i2;
+    ai[j][2] = cd * i2 - sd * r2;
+  }
+  // R23 (.) R23^T
+  const F c = F(sc.c23), s = F(sc.s23);
+  for (int j = 0; j < 3; ++j) {
+    const F r1 = ar[1][j], i1 = ai[1][j], r2 = ar[2][j], i2 = ai[2][j];
+    ar[1][j] = c * r1 + s * r2;
+    ai[1][j] = c * i1 + s * i2;
+    ar[2][j] = c * r2 - s * r1;
+    ai[2][j] = c * i2 - s * i1;
+  }
+  for (int i = 0; i < 3; ++i) {
+    const F r1 = ar[i][1], i1 = ai[i][1], r2 = ar[i][2], i2 = ai[i][2];
+    ar[i][1] = c * r1 + s * r2;
+    ai[i][1] = c * i1 + s * i2;
+    ar[i][2] = c * r2 - s * r1;
+    ai[i][2] = c * i2 - s * i1;
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      prob[a * 3 + b] = ar[b][a] * ar[b][a] + ai[b][a] * ai[b][a];
+}
+
+// A single constant-density layer of length L (the beam path,
+// physics.hpp:1153-1260): Z = I, then the same finish as a trajectory.
+template <class P>
+CP3_HD void beam_point(const SetConst& sc, typename P::D E,
+                       typename P::D rhoYe, typename P::D L,
+                       typename P::F prob[9]) {
+  using D = typename P::D;
+  Slab<P> s;
+  solve_slab<P>(sc, E, D(1) / E, rhoYe, s);
+  Traj<P> t;
+  traj_init<P>(t, true);
+  traj_finish<P>(t, s, L, sc, prob);
+}
+
+}  // namespace cp3

@@ -1,0 +1,162 @@
+/*
+ * C ABI of the MI355X three-flavour oscillation propagator (libcp3hip.so).
+ *
+ * This is the only interface the HIP library exports.  It replaces the
+ * device-facing part of the reference's GPU propagators:
+ *   AtmosCudaPropagatorSingle  (atmoscudapropagator.cuh:42-145,
+ *                               atmoscudapropagator.cu:30-325)
+ *   BeamCudaPropagatorSingle   (beamcudapropagator.cuh:41-121,
+ *                               beamcudapropagator.cu:29-214)
+ * and their physics core (physics.hpp:789-1394).  The C++ classes in
+ * include/cudaprob3_amd/hip_propagator.hpp wrap it with the reference's
+ * method names and exceptions.
+ *
+ * Conventions
+ *   - Every call returns an int status: CP3_OK (0) or a negative CP3_ERR_*.
+ *     Nothing throws or exits across this boundary; cp3_last_error() gives
+ *     the message of the calling thread's last failure.
+ *   - Units are the reference setters': GeV, km, g/cm^3, radians, eV^2.
+ *   - Channel t = a*3+b is P(a->b) with e, mu, tau = 0, 1, 2
+ *     (types.hpp:23-27).  Neutrino type 0 = neutrino, 1 = antineutrino
+ *     (types.hpp:29).
+ *   - Device results are laid out [9][nC][nE] (atmospheric) or [9][nE] (beam),
+ *     as the reference kernels write them (physics.hpp:1139-1140, 1248-1249);
+ *     elements are double for CP3_FP64 and float for CP3_FP32 / CP3_MIXED.
+ *   - One object is used by one host thread at a time; objects share no state,
+ *     so different objects may run concurrently on their own streams.
+ */
+#ifndef CP3_HIP_H
+#define CP3_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision forms. */
+#define CP3_FP64 0  /* FP64 throughout                                   */
+#define CP3_FP32 1  /* FP32 throughout                                   */
+#define CP3_MIXED 2 /* FP64 eigen-solve and phases, FP32 amplitude work  */
+
+/* Status codes. */
+#define CP3_OK 0
+#define CP3_ERR_INVALID (-1) /* bad argument; the reference throws runtime_error */
+#define CP3_ERR_STATE (-2)   /* call order; the reference throws runtime_error */
+#define CP3_ERR_RANGE (-3)   /* index out of range (getProbability)             */
+#define CP3_ERR_DEVICE (-4)  /* HIP runtime failure; the reference exits       */
+#define CP3_ERR_NOMEM (-5)   /* device or pinned allocation failed             */
+#define CP3_ERR_CAPACITY (-6) /* admission refused: memory cap reached         */
+
+typedef struct cp3_atmos cp3_atmos;
+typedef struct cp3_beam cp3_beam;
+
+/* Message of the calling thread's last failing call ("" if none). */
+const char* cp3_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int cp3_version(void);
+/* Number of visible HIP devices. */
+int cp3_device_count(int* n);
+
+/* ---------------------------------------------------------------------------
+ * Atmospheric propagator (one grid, one parameter set per calculate).
+ * ------------------------------------------------------------------------- */
+
+/* AtmosCudaPropagatorSingle(id, n_cosines, n_energies)
+ * (atmoscudapropagator.cu:30-62).  precision: CP3_FP64 / CP3_FP32 /
+ * CP3_MIXED.  Allocates the device result buffer and a stream. */
+int cp3_atmos_create(int device, int n_cosines, int n_energies, int precision,
+                     cp3_atmos** out);
+/* ~AtmosCudaPropagatorSingle (atmoscudapropagator.cu:71-76). */
+int cp3_atmos_destroy(cp3_atmos* p);
+
+/* Propagator::setEnergyList (propagator.hpp:213-218): n must equal
+ * n_energies. */
+int cp3_atmos_set_energy_list(cp3_atmos* p, const double* energies, int n);
+/* AtmosCpuPropagator::setCosineList (atmoscpupropagator.hpp:368-381): n must
+ * equal n_cosines. */
+int cp3_atmos_set_cosine_list(cp3_atmos* p, const double* cosines, int n);
+/* Propagator::setMNSMatrix (propagator.hpp:148-183): delta is negated when
+ * kNuType < 0. */
+int cp3_atmos_set_mns_matrix(cp3_atmos* p, double theta12, double theta13,
+                             double theta23, double dcp, int kNuType);
+/* Propagator::setNeutrinoMasses (propagator.hpp:188-209); dm23sq is
+ * Delta m^2_32. */
+int cp3_atmos_set_neutrino_masses(cp3_atmos* p, double dm12sq, double dm23sq);
+/* AtmosCpuPropagator::setDensity(radii, rhos, yps)
+ * (atmoscpupropagator.hpp:116-170): rows sorted either way, the density on a
+ * row belongs to the shell whose outer radius is that row.  No shell cap. */
+int cp3_atmos_set_density(cp3_atmos* p, const double* radii_km,
+                          const double* rhos, const double* yps, int n);
+/* AtmosCpuPropagator::setDensityFromFile (atmoscpupropagator.hpp:250-344),
+ * 3-column (radius, density, Ye) files; other formats return
+ * CP3_ERR_INVALID. */
+int cp3_atmos_set_density_from_file(cp3_atmos* p, const char* path);
+/* setChemicalComposition (atmoscpupropagator.hpp:481-490): n must equal the
+ * number of model rows; values in the stored (outermost first) order. */
+int cp3_atmos_set_chemical_composition(cp3_atmos* p, const double* yps, int n);
+/* setProductionHeight (atmoscpupropagator.hpp:386-393): km; the cosine list
+ * must be set first. */
+int cp3_atmos_set_production_height(cp3_atmos* p, double height_km);
+/* getNlayerBoundaries (atmoscpupropagator.hpp:85): number of model rows. */
+int cp3_atmos_get_nlayer_boundaries(cp3_atmos* p, int* n);
+
+/* calculateProbabilities (atmoscudapropagator.cu:202-205, 266-316):
+ * launches and waits. */
+int cp3_atmos_calculate(cp3_atmos* p, int nutype);
+/* The launch alone (calculateAtmosProbabilitiesAsync, .cu:266-310) and the
+ * wait (waitForCompletion, .cu:313-316). */
+int cp3_atmos_calculate_async(cp3_atmos* p, int nutype);
+int cp3_atmos_wait(cp3_atmos* p);
+
+/* getProbability(ic, ie, t) (atmoscudapropagator.cu:222-235): the first
+ * access after a calculate copies the whole result to pinned host memory. */
+int cp3_atmos_get_probability(cp3_atmos* p, int index_cosine,
+                              int index_energy, int t, double* out);
+/* getProbabilityArr (atmoscudapropagator.cu:238-257): n_energies*n_cosines
+ * values of channel t, energy outer, cosine inner. */
+int cp3_atmos_get_probability_arr(cp3_atmos* p, double* out, int t);
+/* Device pointer of the [9][nC][nE] result and its element size (8 or 4). */
+int cp3_atmos_result_device(cp3_atmos* p, const void** dptr, int* elem_bytes);
+/* HIP stream (hipStream_t) the object launches on. */
+int cp3_atmos_get_stream(cp3_atmos* p, void** stream);
+/* Milliseconds of the last kernel, measured with HIP events around it. */
+int cp3_atmos_last_kernel_ms(cp3_atmos* p, float* ms);
+/* Cosine-block size of the kernel (1, 2, 4 or 8; 0 = automatic). */
+int cp3_atmos_set_cosine_block(cp3_atmos* p, int nc);
+
+/* ---------------------------------------------------------------------------
+ * Beam propagator (one energy list, one parameter set per calculate).
+ * ------------------------------------------------------------------------- */
+
+/* BeamCudaPropagatorSingle(id, n_energies, n_threads)
+ * (beamcudapropagator.cu:29-59); the thread count has no GPU meaning. */
+int cp3_beam_create(int device, int n_energies, int precision, cp3_beam** out);
+int cp3_beam_destroy(cp3_beam* p);
+int cp3_beam_set_energy_list(cp3_beam* p, const double* energies, int n);
+int cp3_beam_set_mns_matrix(cp3_beam* p, double theta12, double theta13,
+                            double theta23, double dcp, int kNuType);
+int cp3_beam_set_neutrino_masses(cp3_beam* p, double dm12sq, double dm23sq);
+/* BeamCpuPropagator::setDensity (beamcpupropagator.hpp:84-86): g/cm^3;
+ * Ye = 0.5 is applied in the calculation (physics.hpp:1219). */
+int cp3_beam_set_density(cp3_beam* p, double rho);
+/* BeamCpuPropagator::setPathLength (beamcpupropagator.hpp:89-91): km. */
+int cp3_beam_set_path_length(cp3_beam* p, double length_km);
+int cp3_beam_calculate(cp3_beam* p, int nutype);
+int cp3_beam_calculate_async(cp3_beam* p, int nutype);
+int cp3_beam_wait(cp3_beam* p);
+/* getProbability(ie, t) (beamcudapropagator.cu:146-159). */
+int cp3_beam_get_probability(cp3_beam* p, int index_energy, int t,
+                             double* out);
+/* getProbabilityArr (beamcudapropagator.cu:162-178). */
+int cp3_beam_get_probability_arr(cp3_beam* p, double* out, int t);
+int cp3_beam_result_device(cp3_beam* p, const void** dptr, int* elem_bytes);
+int cp3_beam_get_stream(cp3_beam* p, void** stream);
+int cp3_beam_last_kernel_ms(cp3_beam* p, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CP3_HIP_H */

@@ -1,0 +1,707 @@
+// ORACLE - test infrastructure only.
+//
+// A CPU restatement, in plain double-precision C++, of the reference's
+// three-flavour Barger propagator as run by AtmosCpuPropagator<double> and
+// BeamCpuPropagator<double>.  It exists to check the HIP path: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+// as the checker.  It is never on the product path.
+//
+// Each function follows the reference operation for operation (the same
+// formulas, evaluation order and branches) so that it agrees with the compiled
+// reference to the last bit or nearly so; the reference lines are cited on each
+// function.  Differences on purpose:
+//   * no MaxNLayers cap (constants.hpp:46): shell scratch is sized per call;
+//   * the innermost-shell lookup of radii[i+1] (physics.hpp:777) reads 0 when a
+//     model is passed without its 0-km row, instead of reading past the end;
+//     the value is unused on that branch (physics.hpp:780-786);
+//   * unsupported density files raise an error instead of std::terminate
+//     (atmoscpupropagator.hpp:338-342).
+//
+// Pins (tests/): golden vectors made by the compiled reference
+// (tests/golden/), the reference library itself where it is built
+// (oracle/_ref/), closed forms (vacuum oscillation, U.U^+ = I, eigenvalues vs
+// numpy.linalg.eigvalsh, one layer vs scipy.linalg.expm, chord geometry sums)
+// and invariants (unitarity, nu-bar <-> (delta, A) -> (-delta, -A), T-symmetry
+// at delta = 0).
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+struct cx {
+  double re;
+  double im;
+};
+
+// constants.hpp:28-40 and the hard-coded L/E factor of physics.hpp:444, 524.
+constexpr double kGf2 = 1.52588e-4;  // 2*sqrt(2)*G_F*N_A*(hbar c)^3 [eV^2/GeV/(g/cm^3)]
+constexpr double kKm2Cm = 1.0e5;
+constexpr double kREarthKm = 6371.0;
+constexpr double kREarthCm = kREarthKm * kKm2Cm;
+constexpr double kBeamYe = 0.5;
+constexpr double kLoE = 2.534;
+
+// The state Propagator<double> carries into the physics core:
+// Mix_U, dm (propagator.hpp:279-280) and the vacuum ordering computed by
+// prepare_getMfast (physics.hpp:198-252).
+struct Osc {
+  cx U[3][3];
+  double DM[3][3];
+  int order[3];
+};
+
+// propagator.hpp:148-183: PDG parametrisation, delta negated when kNuType < 0.
+void set_pmns(Osc& o, double t12, double t13, double t23, double dcp,
+              int kNuType) {
+  if (kNuType < 0) dcp *= -1;
+  const double s12 = std::sin(t12), s13 = std::sin(t13), s23 = std::sin(t23);
+  const double c12 = std::cos(t12), c13 = std::cos(t13), c23 = std::cos(t23);
+  const double sd = std::sin(dcp), cd = std::cos(dcp);
+  o.U[0][0] = {c12 * c13, 0.0};
+  o.U[0][1] = {s12 * c13, 0.0};
+  o.U[0][2] = {s13 * cd, -s13 * sd};
+  o.U[1][0] = {-s12 * c23 - c12 * s23 * s13 * cd, -c12 * s23 * s13 * sd};
+  o.U[1][1] = {c12 * c23 - s12 * s23 * s13 * cd, -s12 * s23 * s13 * sd};
+  o.U[1][2] = {s23 * c13, 0.0};
+  o.U[2][0] = {s12 * s23 - c12 * c23 * s13 * cd, -c12 * c23 * s13 * sd};
+  o.U[2][1] = {-c12 * s23 - s12 * c23 * s13 * cd, -s12 * c23 * s13 * sd};
+  o.U[2][2] = {c23 * c13, 0.0};
+}
+
+// propagator.hpp:188-209: vacuum masses (0, dm12, dm12+dm23) with the 5e-9
+// degeneracy breaker, DM[i][j] = m_i - m_j.
+void set_masses(Osc& o, double dm12sq, double dm23sq) {
+  double m[3] = {0.0, dm12sq, dm12sq + dm23sq};
+  const double brk = 5.0e-9;
+  if (dm12sq == 0.0) m[0] -= brk;
+  if (dm23sq == 0.0) m[2] += brk;
+  o.DM[0][0] = o.DM[1][1] = o.DM[2][2] = 0.0;
+  o.DM[0][1] = m[0] - m[1];
+  o.DM[1][0] = -o.DM[0][1];
+  o.DM[0][2] = m[0] - m[2];
+  o.DM[2][0] = -o.DM[0][2];
+  o.DM[1][2] = m[1] - m[2];
+  o.DM[2][1] = -o.DM[1][2];
+}
+
+// Trigonometric roots of Barger et al. eq. (21) for the coefficients
+// (alpha, beta, gamma); shared shape of physics.hpp:213-229 and 296-323.
+// `clamp_tmp` reproduces getMfast's max(tmp, 0) (physics.hpp:297), which the
+// vacuum version does not apply (physics.hpp:214).
+static void cubic_roots(double alpha, double beta, double gamma, double DM00,
+                        bool clamp_tmp, double root[3]) {
+  double tmp = alpha * alpha - 3.0 * beta;
+  if (clamp_tmp && tmp < 0) tmp = 0;
+  double arg = (2.0 * alpha * alpha * alpha - 9.0 * alpha * beta + 27.0 * gamma) /
+               (2.0 * std::sqrt(tmp * tmp * tmp));
+  if (std::fabs(arg) > 1.0) arg = arg / std::fabs(arg);
+  const double th0 = std::acos(arg) / 3.0;
+  const double th1 = th0 - (2.0 * M_PI / 3.0);
+  const double th2 = th0 + (2.0 * M_PI / 3.0);
+  const double amp = -(2.0 / 3.0) * std::sqrt(tmp);
+  root[0] = amp;
+  root[1] = amp;
+  root[2] = amp;
+  root[0] *= std::cos(th0);
+  root[1] *= std::cos(th1);
+  root[2] *= std::cos(th2);
+  const double shift = DM00 - alpha / 3.0;
+  root[0] += shift;
+  root[1] += shift;
+  root[2] += shift;
+}
+
+// physics.hpp:198-252: order[i] = index of the vacuum root closest to DM(i,0)
+// (first minimum wins).
+void set_order(Osc& o) {
+  const double a = o.DM[0][1] + o.DM[0][2];
+  const double b = o.DM[0][1] * o.DM[0][2];
+  double r[3];
+  cubic_roots(a, b, 0.0, o.DM[0][0], false, r);
+  for (int i = 0; i < 3; ++i) {
+    double best = std::fabs(o.DM[i][0] - r[0]);
+    int k = 0;
+    for (int j = 1; j < 3; ++j) {
+      const double d = std::fabs(o.DM[i][0] - r[j]);
+      if (d < best) {
+        k = j;
+        best = d;
+      }
+    }
+    o.order[i] = k;
+  }
+}
+
+// physics.hpp:276-281, 352-357: matter term, negative for neutrinos.
+static inline double matter_fac(double E, double rhoYe, bool anti) {
+  return anti ? kGf2 * E * rhoYe : -kGf2 * E * rhoYe;
+}
+
+// physics.hpp:267-340 (getMfast): matter eigenvalues ordered by continuity to
+// vacuum; dMM[i][j] = l_i - l_j, dMV[i][j] = l_i - DM(j,0).
+void matter_masses(const Osc& o, double E, double rhoYe, bool anti,
+                   double dMM[3][3], double dMV[3][3]) {
+  const double fac = matter_fac(E, rhoYe, anti);
+  const cx* U0 = o.U[0];
+  const double d01 = o.DM[0][1], d02 = o.DM[0][2];
+  const double alpha = fac + d01 + d02;
+  const double beta =
+      d01 * d02 +
+      fac * (d01 * (1.0 - U0[1].re * U0[1].re - U0[1].im * U0[1].im) +
+             d02 * (1.0 - U0[2].re * U0[2].re - U0[2].im * U0[2].im));
+  const double gamma =
+      fac * d01 * d02 * (U0[0].re * U0[0].re + U0[0].im * U0[0].im);
+  double r[3], lam[3];
+  cubic_roots(alpha, beta, gamma, o.DM[0][0], true, r);
+  for (int i = 0; i < 3; ++i) lam[i] = r[o.order[i]];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      dMM[i][j] = lam[i] - lam[j];
+      dMV[i][j] = lam[i] - o.DM[j][0];
+    }
+}
+
+// physics.hpp:345-430 (get_product): Sylvester projectors of the mass-basis
+// matrix, P[i][j][k] = prod_{m != k} (M - l_m)[i][j] / prod_{m != k}(l_k - l_m).
+void projectors(const Osc& o, double E, double rhoYe, bool anti,
+                const double dMV[3][3], const double dMM[3][3],
+                cx P[3][3][3]) {
+  const double fac = matter_fac(E, rhoYe, anti);
+  cx H[3][3][3];  // H[n][m][j] = (M - l_j)[n][m]
+  for (int n = 0; n < 3; ++n)
+    for (int m = 0; m < 3; ++m) {
+      const cx& a = o.U[0][n];
+      const cx& b = o.U[0][m];
+      const double re = -fac * (a.re * b.re + a.im * b.im);
+      const double im = -fac * (a.re * b.im - a.im * b.re);
+      for (int j = 0; j < 3; ++j) H[n][m][j] = {re, im};
+    }
+  for (int j = 0; j < 3; ++j)
+    for (int n = 0; n < 3; ++n) H[n][n][j].re -= dMV[j][n];
+
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) P[i][j][k] = {0.0, 0.0};
+  // projector k uses the two factors (M - l_a)(M - l_b) with {a,b} = the
+  // other two indices in cyclic order (1,2), (2,0), (0,1).
+  static const int fa[3] = {1, 2, 0};
+  static const int fb[3] = {2, 0, 1};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      for (int m = 0; m < 3; ++m)
+        for (int k = 0; k < 3; ++k) {
+          const cx& x = H[i][m][fa[k]];
+          const cx& y = H[m][j][fb[k]];
+          P[i][j][k].re += x.re * y.re - x.im * y.im;
+          P[i][j][k].im += x.re * y.im + x.im * y.re;
+        }
+      const double d0 = dMM[0][1] * dMM[0][2];
+      const double d1 = dMM[1][2] * dMM[1][0];
+      const double d2 = dMM[2][0] * dMM[2][1];
+      P[i][j][0].re /= d0;
+      P[i][j][0].im /= d0;
+      P[i][j][1].re /= d1;
+      P[i][j][1].im /= d1;
+      P[i][j][2].re /= d2;
+      P[i][j][2].im /= d2;
+    }
+}
+
+// physics.hpp:439-450 (getArg): phases -2.534 * l_k * L / E (phase offset 0).
+void phases(double L, double E, const double dMV[3][3], double arg[3]) {
+  for (int k = 0; k < 3; ++k) arg[k] = -kLoE * dMV[k][0] * L / E;
+}
+
+// physics.hpp:459-514 (getC): C_k = U P_k U^+ in the flavour basis.
+void flavour_projectors(const Osc& o, const cx P[3][3][3], cx C[3][3][3]) {
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a) {
+      double rr[3] = {0., 0., 0.}, ri[3] = {0., 0., 0.};
+      double ir[3] = {0., 0., 0.}, ii[3] = {0., 0., 0.};
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          rr[j] += o.U[a][i].re * P[i][j][k].re;
+          ri[j] += o.U[a][i].re * P[i][j][k].im;
+          ir[j] += o.U[a][i].im * P[i][j][k].re;
+          ii[j] += o.U[a][i].im * P[i][j][k].im;
+        }
+      for (int b = 0; b < 3; ++b) {
+        double sre = 0., sim = 0.;
+        for (int j = 0; j < 3; ++j) {
+          const cx& u = o.U[b][j];
+          sre += rr[j] * u.re;
+          sre += ri[j] * u.im;
+          sre += ir[j] * u.im;
+          sre -= ii[j] * u.re;
+          sim += ii[j] * u.im;
+          sim += ir[j] * u.re;
+          sim += ri[j] * u.re;
+          sim -= rr[j] * u.im;
+        }
+        C[k][a][b] = {sre, sim};
+      }
+    }
+}
+
+// physics.hpp:620-629 (get_transition_matrix_expansion).
+void layer_expansion(const Osc& o, double E, double rhoYe, double L_km,
+                     bool anti, cx C[3][3][3], double arg[3]) {
+  double dMM[3][3], dMV[3][3];
+  cx P[3][3][3];
+  matter_masses(o, E, rhoYe, anti, dMM, dMV);
+  phases(L_km, E, dMV, arg);
+  projectors(o, E, rhoYe, anti, dMV, dMM, P);
+  flavour_projectors(o, P, C);
+}
+
+// math.hpp:57-75 (multiply_phase_matrix, host branch): B += A * e^{i phase}.
+static inline void add_phased(double phase, const cx A[3][3], cx B[3][3]) {
+  const double s = std::sin(phase);
+  const double c = std::cos(phase);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      B[i][j].re += c * A[i][j].re - s * A[i][j].im;
+      B[i][j].im += c * A[i][j].im + s * A[i][j].re;
+    }
+}
+
+// math.hpp:82-96 (multiply_complex_matrix): C += A * B.
+static inline void mul_acc(const cx A[3][3], const cx B[3][3], cx C[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        C[i][j].re += A[i][k].re * B[k][j].re - A[i][k].im * B[k][j].im;
+        C[i][j].im += A[i][k].im * B[k][j].re + A[i][k].re * B[k][j].im;
+      }
+}
+
+static inline void zero(cx A[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) A[i][j] = {0.0, 0.0};
+}
+
+// C = A * B into a fresh matrix (clear + multiply, physics.hpp:972-974).
+static inline void mul(const cx A[3][3], const cx B[3][3], cx C[3][3]) {
+  zero(C);
+  mul_acc(A, B, C);
+}
+
+// physics.hpp:745-787 (getTraversedDistanceOfLayer), in cm.  `radii` is
+// descending with n entries; radii[n] reads as 0 (see header).
+double segment_cm(const double* radii, int n, int layer, int max_layer,
+                  double path_cm, double earth_cm, double cz) {
+  if (cz >= 0) return path_cm;
+  if (layer == 0) return path_cm - earth_cm;
+  const int i = (layer <= max_layer) ? layer - 1 : 2 * max_layer - layer - 1;
+  const double r2min = kREarthKm * kREarthKm * (1 - cz * cz);
+  const double ri = radii[i];
+  const double rn = (i + 1 < n) ? radii[i + 1] : 0.0;
+  double cross_this = 2.0 * std::sqrt(ri * ri - r2min);
+  if (std::sqrt(r2min) > ri) cross_this = 0;
+  double cross_next = 2.0 * std::sqrt(rn * rn - r2min);
+  if (std::sqrt(r2min) > rn) cross_next = 0;
+  if (i < max_layer - 1) return 0.5 * (cross_this - cross_next) * kKm2Cm;
+  return cross_this * kKm2Cm;
+}
+
+// physics.hpp:634-646 (getDensityOfLayer): rho*Ye of the shell, 0 in vacuum.
+double layer_rho_ye(const double* rhos, const double* yps, int layer,
+                    int max_layer) {
+  if (layer == 0) return 0.0;
+  const int i = (layer <= max_layer) ? layer - 1 : 2 * max_layer - layer - 1;
+  return rhos[i] * yps[i];
+}
+
+// An Earth model as AtmosCpuPropagator<double> holds it after setDensity
+// (atmoscpupropagator.hpp:116-170): rows descending, coslimit per row.
+struct Earth {
+  std::vector<double> radii, rhos, yps, coslimit;
+};
+
+// atmoscpupropagator.hpp:116-170.
+void set_density(Earth& e, const std::vector<double>& r,
+                 const std::vector<double>& rho,
+                 const std::vector<double>& ye) {
+  if (rho.size() != r.size())
+    throw std::runtime_error("setDensity : rhos.size() != radii.size()");
+  if (rho.size() != ye.size())
+    throw std::runtime_error("setDensity : rhos.size() != yps.size()");
+  if (rho.empty() || r.empty() || ye.empty())
+    throw std::runtime_error("setDensity : vectors must not be empty");
+  bool flip = false;
+  if (r.size() >= 2) {
+    const int sign = (r[1] - r[0] > 0 ? 1 : -1);
+    for (size_t i = 1; i < r.size(); ++i)
+      if ((r[i] - r[i - 1]) * sign < 0)
+        throw std::runtime_error("radii order messed up");
+    flip = (sign == 1);
+  }
+  e.radii = r;
+  e.rhos = rho;
+  e.yps = ye;
+  if (flip) {
+    std::vector<double> a(e.radii.rbegin(), e.radii.rend());
+    std::vector<double> b(e.rhos.rbegin(), e.rhos.rend());
+    std::vector<double> c(e.yps.rbegin(), e.yps.rend());
+    e.radii.swap(a);
+    e.rhos.swap(b);
+    e.yps.swap(c);
+  }
+  e.coslimit.clear();
+  for (size_t i = 0; i < e.radii.size(); ++i) {
+    double x = -1 * std::sqrt(1 - (e.radii[i] * e.radii[i] /
+                                   (kREarthKm * kREarthKm)));
+    if (i == 0) x = 0;
+    e.coslimit.push_back(x);
+  }
+}
+
+// atmoscpupropagator.hpp:431-444 without the MaxNLayers cap: number of shell
+// rows whose cosine limit lies strictly above c.
+int max_layer(const Earth& e, double c) {
+  int n = 0;
+  for (double lim : e.coslimit)
+    if (c < lim) ++n;
+  return n;
+}
+
+// Splits a density-file line on single or repeated spaces the way
+// atmoscpupropagator.hpp:265-281 / 313-322 does, returning the field count.
+static std::vector<std::string> split_fields(std::string line) {
+  std::vector<std::string> f;
+  while (line.find_first_of(" ") != std::string::npos) {
+    size_t pos = line.find_first_of(" ");
+    f.push_back(line.substr(0, pos));
+    while (pos + 1 < line.size() && line[pos] == line[pos + 1]) ++pos;
+    line = line.substr(pos + 1);
+  }
+  f.push_back(line);
+  return f;
+}
+
+// atmoscpupropagator.hpp:250-344 for the 3-column (radius, rho, Ye) format.
+// The 5-column polynomial format is outside this oracle's scope and raises.
+void set_density_file(Earth& e, const std::string& path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("could not open density file " + path);
+  std::string line;
+  int ncol = 0;
+  while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    const int n = (int)split_fields(line).size();
+    if (ncol == 0) ncol = n;
+  }
+  if (ncol != 3)
+    throw std::runtime_error("unsupported Earth model: " +
+                             std::to_string(ncol) + " entries per line");
+  in.clear();
+  in.seekg(0);
+  std::vector<double> r, d, y;
+  double a, b, c;
+  while (in >> a >> b >> c) {
+    r.push_back(a);
+    d.push_back(b);
+    y.push_back(c);
+  }
+  set_density(e, r, d, y);
+}
+
+// physics.hpp:789-1150 (calculate_atmos, constant-density, no production
+// height averaging).  out is [cos][E][9], channel a*3+b = P(a->b).
+void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
+                     const double* cosv, int nC, const double* ev, int nE,
+                     double h_cm, double* out) {
+  const int nrows = (int)earth.radii.size();
+  std::vector<int> maxl((size_t)nC);
+  for (int c = 0; c < nC; ++c) maxl[(size_t)c] = max_layer(earth, cosv[c]);
+
+#pragma omp parallel for schedule(dynamic)
+  for (int ic = 0; ic < nC; ++ic) {
+    const double cz = cosv[ic];
+    const double earth_cm = -2.0 * cz * kREarthCm;
+    const int M = maxl[(size_t)ic];
+    cx C_atm[3][3][3];
+    for (int ie = 0; ie < nE; ++ie) {
+      const double E = ev[ie];
+      cx fin[3][3], ctm[3][3], T[3][3], tmp[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) ctm[i][j] = {i == j ? 1.0 : 0.0, 0.0};
+      const double path_cm =
+          std::sqrt((kREarthCm + h_cm) * (kREarthCm + h_cm) -
+                    (kREarthCm * kREarthCm) * (1 - cz * cz)) -
+          kREarthCm * cz;
+      for (int layer = 0; layer <= M; ++layer) {
+        const double dist = segment_cm(earth.radii.data(), nrows, layer, M,
+                                       path_cm, earth_cm, cz);
+        const double rye =
+            layer_rho_ye(earth.rhos.data(), earth.yps.data(), layer, M);
+        cx C[3][3][3];
+        double arg[3];
+        layer_expansion(o, E, rye, dist / kKm2Cm, anti, C, arg);
+        zero(T);
+        for (int k = 0; k < 3; ++k) add_phased(arg[k], C[k], T);
+        if (layer == 0) {
+          std::memcpy(fin, T, sizeof(fin));
+          std::memcpy(C_atm, C, sizeof(C_atm));
+        } else if (layer < M) {
+          mul(T, fin, tmp);
+          std::memcpy(fin, tmp, sizeof(fin));
+          mul(ctm, T, tmp);
+          std::memcpy(ctm, tmp, sizeof(ctm));
+        } else {
+          mul(T, fin, tmp);
+          std::memcpy(fin, tmp, sizeof(fin));
+        }
+      }
+      mul(ctm, fin, tmp);
+      std::memcpy(fin, tmp, sizeof(fin));
+
+      // physics.hpp:1102-1130: P(a->b) assembled from the expansion of the
+      // atmosphere layer; the length-shift factor is 1 without averaging.
+      cx prod[3][3][3];
+      for (int k = 0; k < 3; ++k) mul(fin, C_atm[k], prod[k]);
+      double prob[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+      for (int k = 0; k < 3; ++k) {
+        for (int b = 0; b < 3; ++b)
+          for (int a = 0; a < 3; ++a)
+            prob[a][b] += prod[k][a][b].re * prod[k][a][b].re +
+                          prod[k][a][b].im * prod[k][a][b].im;
+        for (int j = 0; j < k; ++j)
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a)
+              prob[a][b] += 2. * prod[j][a][b].re * prod[k][a][b].re * 1.0 +
+                            2. * prod[j][a][b].im * prod[k][a][b].im * 1.0 +
+                            2. * prod[j][a][b].im * prod[k][a][b].re * 0.0 -
+                            2. * prod[j][a][b].re * prod[k][a][b].im * 0.0;
+      }
+      double* dst = out + ((size_t)ic * nE + ie) * 9;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) dst[a * 3 + b] = prob[b][a];
+    }
+  }
+}
+
+// physics.hpp:516-598 (getA): beam amplitude U X U^+ with
+// X = sum_k P_k e^{i arg_k}, contracted through the AXFAC factors of
+// physics.hpp:157-174.
+void beam_amplitude(const Osc& o, double E, double rhoYe, double L_km,
+                    bool anti, cx A[3][3]) {
+  double dMM[3][3], dMV[3][3];
+  cx P[3][3][3];
+  matter_masses(o, E, rhoYe, anti, dMM, dMV);
+  projectors(o, E, rhoYe, anti, dMV, dMM, P);
+  cx X[3][3];
+  zero(X);
+  for (int k = 0; k < 3; ++k) {
+    const double arg = -kLoE * dMV[k][0] * L_km / E;
+    const double s = std::sin(arg);
+    const double c = std::cos(arg);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        X[i][j].re += c * P[i][j][k].re - s * P[i][j][k].im;
+        X[i][j].im += c * P[i][j][k].im + s * P[i][j][k].re;
+      }
+  }
+  for (int n = 0; n < 3; ++n)
+    for (int m = 0; m < 3; ++m) {
+      double are = 0, aim = 0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const cx& u = o.U[n][i];
+          const cx& v = o.U[m][j];
+          const double f0 = u.re * v.re + u.im * v.im;
+          const double f1 = u.re * v.im - u.im * v.re;
+          const double f2 = u.im * v.im + u.re * v.re;
+          const double f3 = u.im * v.re - u.re * v.im;
+          are += f0 * X[i][j].re + f1 * X[i][j].im;
+          aim += f2 * X[i][j].im + f3 * X[i][j].re;
+        }
+      A[n][m] = {are, aim};
+    }
+}
+
+// physics.hpp:1153-1260 (calculate_beam).  out is [E][9].
+void calculate_beam(const Osc& o, bool anti, const double* ev, int nE,
+                    double rho, double L_km, double* out) {
+#pragma omp parallel for
+  for (int ie = 0; ie < nE; ++ie) {
+    cx A[3][3];
+    beam_amplitude(o, ev[ie], rho * kBeamYe, L_km, anti, A);
+    double* dst = out + (size_t)ie * 9;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        dst[a * 3 + b] = 0. + (A[b][a].re * A[b][a].re + A[b][a].im * A[b][a].im);
+  }
+}
+
+static void make_osc(Osc& o, double t12, double t13, double t23, double dcp,
+                     int kNuType, double dm12, double dm23) {
+  set_pmns(o, t12, t13, t23, dcp, kNuType);
+  set_masses(o, dm12, dm23);
+  set_order(o);
+}
+
+static void put_err(char* err, int errlen, const char* msg) {
+  if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", msg);
+}
+
+static void set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#else
+  (void)threads;
+#endif
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Atmospheric probabilities, [cos][E][9].  Argument meaning follows the
+// reference call sequence (example/mainatmos.cpp:121-148).
+int orc_atmos_f64(int nutype, const double* cosv, int nC, const double* ev,
+                  int nE, const double* radii, const double* rhos,
+                  const double* yps, int nrows, double th12, double th13,
+                  double th23, double dcp, int kNuType, double dm12,
+                  double dm23, double h_km, int threads, double* out,
+                  char* err, int errlen) {
+  try {
+    orc::Osc o;
+    orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+    orc::Earth e;
+    orc::set_density(e, std::vector<double>(radii, radii + nrows),
+                     std::vector<double>(rhos, rhos + nrows),
+                     std::vector<double>(yps, yps + nrows));
+    orc::set_threads(threads);
+    orc::calculate_atmos(o, e, nutype != 0, cosv, nC, ev, nE,
+                         h_km * 100000.0, out);
+  } catch (const std::exception& ex) {
+    orc::put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+int orc_atmos_file_f64(int nutype, const double* cosv, int nC,
+                       const double* ev, int nE, const char* path, double th12,
+                       double th13, double th23, double dcp, int kNuType,
+                       double dm12, double dm23, double h_km, int threads,
+                       double* out, char* err, int errlen) {
+  try {
+    orc::Osc o;
+    orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+    orc::Earth e;
+    orc::set_density_file(e, path);
+    orc::set_threads(threads);
+    orc::calculate_atmos(o, e, nutype != 0, cosv, nC, ev, nE,
+                         h_km * 100000.0, out);
+  } catch (const std::exception& ex) {
+    orc::put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// Beam probabilities, [E][9]; rho in g/cm^3 (Ye = 0.5 applied), L in km.
+int orc_beam_f64(int nutype, const double* ev, int nE, double rho,
+                 double L_km, double th12, double th13, double th23,
+                 double dcp, int kNuType, double dm12, double dm23,
+                 int threads, double* out, char* err, int errlen) {
+  try {
+    orc::Osc o;
+    orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+    orc::set_threads(threads);
+    orc::calculate_beam(o, nutype != 0, ev, nE, rho, L_km, out);
+  } catch (const std::exception& ex) {
+    orc::put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// --- single-function probes for the closed-form tests -------------------
+
+// U as [3][3] complex (18 doubles, re/im interleaved).
+void orc_pmns(double th12, double th13, double th23, double dcp, int kNuType,
+              double* U18) {
+  orc::Osc o;
+  orc::set_pmns(o, th12, th13, th23, dcp, kNuType);
+  std::memcpy(U18, o.U, sizeof(o.U));
+}
+
+// Ordered matter eigenvalues l_i (mass-basis matrix M, eV^2) and the vacuum
+// ordering.
+void orc_matter_eigs(double th12, double th13, double th23, double dcp,
+                     int kNuType, double dm12, double dm23, double E,
+                     double rhoYe, int nutype, double* lam3, int* order3) {
+  orc::Osc o;
+  orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+  double dMM[3][3], dMV[3][3];
+  orc::matter_masses(o, E, rhoYe, nutype != 0, dMM, dMV);
+  for (int i = 0; i < 3; ++i) lam3[i] = dMV[i][0] + o.DM[0][0];
+  for (int i = 0; i < 3; ++i) order3[i] = o.order[i];
+}
+
+// One layer's transition matrix T = sum_k C_k e^{i arg_k} (flavour basis,
+// [after][before], 18 doubles) and the mass-basis projectors P (54 doubles,
+// [i][j][k]).
+void orc_layer(double th12, double th13, double th23, double dcp, int kNuType,
+               double dm12, double dm23, double E, double rhoYe, double L_km,
+               int nutype, double* T18, double* P54) {
+  orc::Osc o;
+  orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+  orc::cx C[3][3][3], T[3][3], P[3][3][3];
+  double arg[3], dMM[3][3], dMV[3][3];
+  orc::layer_expansion(o, E, rhoYe, L_km, nutype != 0, C, arg);
+  orc::zero(T);
+  for (int k = 0; k < 3; ++k) orc::add_phased(arg[k], C[k], T);
+  std::memcpy(T18, T, sizeof(T));
+  orc::matter_masses(o, E, rhoYe, nutype != 0, dMM, dMV);
+  orc::projectors(o, E, rhoYe, nutype != 0, dMV, dMM, P);
+  std::memcpy(P54, P, sizeof(P));
+}
+
+// Geometry of one trajectory: max layer index M and the 2M+1 segment lengths
+// (km) walked by the reference layer loop (layer 0 = atmosphere, then shells
+// inward to the innermost full chord, listed once each as the loop visits
+// them, i.e. M+1 values in seg_km[0..M]).  Returns M, or -1 on bad input.
+int orc_segments(double cz, const double* radii, const double* rhos,
+                 const double* yps, int nrows, double h_km, double* seg_km,
+                 double* rho_ye, int cap) {
+  try {
+    orc::Earth e;
+    orc::set_density(e, std::vector<double>(radii, radii + nrows),
+                     std::vector<double>(rhos, rhos + nrows),
+                     std::vector<double>(yps, yps + nrows));
+    const int M = orc::max_layer(e, cz);
+    if (M + 1 > cap) return -1;
+    const double h_cm = h_km * 100000.0;
+    const double path_cm =
+        std::sqrt((orc::kREarthCm + h_cm) * (orc::kREarthCm + h_cm) -
+                  (orc::kREarthCm * orc::kREarthCm) * (1 - cz * cz)) -
+        orc::kREarthCm * cz;
+    const double earth_cm = -2.0 * cz * orc::kREarthCm;
+    for (int l = 0; l <= M; ++l) {
+      seg_km[l] = orc::segment_cm(e.radii.data(), nrows, l, M, path_cm,
+                                  earth_cm, cz) /
+                  orc::kKm2Cm;
+      rho_ye[l] = orc::layer_rho_ye(e.rhos.data(), e.yps.data(), l, M);
+    }
+    return M;
+  } catch (...) {
+    return -1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+"""Parity of the HIP kernels (through the C ABI) with the reference.
+
+FP64: within 1e-10 absolute per probability of the reference CPU propagator
+(golden vectors from the compiled reference) and of the oracle on the same
+inputs, with unitarity checked.  FP32 and mixed: error bounds against the FP64
+oracle (mixed: 1e-5 max absolute).
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, osc_for
+from cudaprob3_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+TOL_MIXED = 1e-5
+TOL_FP32 = 2e-3
+
+
+def run_atmos(cz, ev, radii, rhos, yps, params, nutype, precision="fp64",
+              h_km=22.0, nc=0):
+    from cudaprob3_amd import AtmosPropagator
+    with AtmosPropagator(len(cz), len(ev), precision=precision) as p:
+        if nc:
+            p.setCosineBlock(nc)
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        t12, t13, t23, d, m12, m23 = [float(x) for x in params]
+        p.setMNSMatrix(t12, t13, t23, d, -1 if nutype else 1)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(radii, rhos, yps)
+        p.setProductionHeight(h_km)
+        p.calculateProbabilities(nutype)
+        return p.probabilities()
+
+
+def run_beam(ev, params, nutype, precision="fp64", rho=2.6, L=1300.0):
+    from cudaprob3_amd import BeamPropagator
+    with BeamPropagator(len(ev), precision=precision) as p:
+        p.setEnergyList(ev)
+        t12, t13, t23, d, m12, m23 = [float(x) for x in params]
+        p.setMNSMatrix(t12, t13, t23, d, -1 if nutype else 1)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(rho)
+        p.setPathLength(L)
+        p.calculateProbabilities(nutype)
+        return p.probabilities()
+
+
+NOM = [W.TH12, W.TH13, W.TH23, W.DCP, W.DM12, W.DM23]
+
+
+def unitarity(prob):
+    P = prob.reshape(-1, 3, 3)
+    return max(np.abs(P.sum(-1) - 1).max(), np.abs(P.sum(-2) - 1).max())
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_prem4_fp64_vs_golden(nutype):
+    g = golden(f"g0_prem4_{'nubar' if nutype else 'nu'}")
+    out = run_atmos(g["cosv"], g["ev"], g["radii"], g["rhos"], g["yps"], NOM,
+                    nutype)
+    s = int(g["stride"])
+    print("prem4 nutype", nutype, "max|d| vs golden",
+          np.abs(out[::s, ::s] - g["prob"]).max())
+    assert np.abs(out[::s, ::s] - g["prob"]).max() < TOL_FP64
+    assert np.allclose(out.sum(axis=(0, 1)), g["chan_sum"], rtol=0,
+                       atol=TOL_FP64 * out.shape[0] * out.shape[1])
+    assert unitarity(out) < 1e-12
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_prem4_fp64_vs_oracle_full(oracle, nutype):
+    e = W.prem4()
+    cz, ev = W.cos_grid(100), W.log_grid(100, 0.1, 100.0)
+    out = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, nutype)
+    ref = oracle.atmos(nutype, cz, ev, e.radii, e.rhos, e.yps,
+                       osc_for(NOM, nutype), 22.0, threads=4)
+    assert np.abs(out - ref).max() < TOL_FP64
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_shells12_fp64_vs_golden(nutype):
+    g = golden(f"g1_shells12_{'nubar' if nutype else 'nu'}")
+    out = run_atmos(g["cosv"], g["ev"], g["radii"], g["rhos"], g["yps"], NOM,
+                    nutype)
+    assert np.abs(out - g["prob"]).max() < TOL_FP64
+    assert unitarity(out) < 1e-12
+
+
+def test_shells59_fp64_vs_golden():
+    g = golden("g3_shells59")
+    for nt in (0, 1):
+        for i, par in enumerate(g["params"]):
+            out = run_atmos(g["cosv"], g["ev"], g["radii"], g["rhos"],
+                            g["yps"], par, nt)
+            assert np.abs(out - g["prob"][nt, i]).max() < TOL_FP64
+
+
+def test_edges_fp64_vs_golden():
+    g = golden("g4_edges")
+    for key in ["nominal", "dm12_zero", "dm23_zero", "dcp_zero", "inverted"]:
+        for nt in (0, 1):
+            out = run_atmos(g["cosv"], g["ev"], g["radii"], g["rhos"],
+                            g["yps"], g[f"params_{key}"], nt)
+            ref = g[f"prob_{key}_{nt}"]
+            assert np.abs(out - ref).max() < TOL_FP64, key
+
+
+def test_beam_fp64_vs_golden():
+    g = golden("g2_beam")
+    for nt in (0, 1):
+        for i, par in enumerate(g["params"]):
+            out = run_beam(g["ev"], par, nt)
+            assert np.abs(out - g["prob"][nt, i]).max() < TOL_FP64
+            assert unitarity(out) < 1e-12
+
+
+@pytest.mark.parametrize("nc", [1, 2, 4, 8])
+def test_cosine_blocks_identical(nc):
+    """The cosine-block size changes only which lanes share an eigen-solve,
+    never a point's arithmetic: results are bitwise identical."""
+    e = W.earth_12()
+    cz, ev = W.cos_grid(37), W.log_grid(70, 0.1, 100.0)
+    base = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, nc=1)
+    out = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, nc=nc)
+    assert np.array_equal(base, out)
+    for prec in ("fp32", "mixed"):
+        b = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=1)
+        o = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=nc)
+        assert np.array_equal(b, o)
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_mixed_and_fp32_error(oracle, nutype):
+    e = W.earth_12()
+    cz, ev = W.cos_grid(120), W.log_grid(120, 0.1, 100.0)
+    ref = oracle.atmos(nutype, cz, ev, e.radii, e.rhos, e.yps,
+                       osc_for(NOM, nutype), 22.0, threads=4)
+    mixed = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, nutype, "mixed")
+    fp32 = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, nutype, "fp32")
+    dm, d32 = np.abs(mixed - ref), np.abs(fp32 - ref)
+    print(f"nutype {nutype}: mixed max {dm.max():.2e} rms "
+          f"{np.sqrt((dm**2).mean()):.2e}; fp32 max {d32.max():.2e} rms "
+          f"{np.sqrt((d32**2).mean()):.2e}")
+    assert dm.max() < TOL_MIXED
+    assert d32.max() < TOL_FP32
+
+
+def test_beam_mixed_error(oracle):
+    ev = W.log_grid(3000, 0.1, 20.0)
+    for par in W.draw_param_sets(3, 5):
+        row = [par.th12, par.th13, par.th23, par.dcp, par.dm12, par.dm23]
+        for nt in (0, 1):
+            ref = oracle.beam(nt, ev, 2.6, 1300.0, osc_for(row, nt))
+            out = run_beam(ev, row, nt, "mixed")
+            assert np.abs(out - ref).max() < TOL_MIXED

@@ -1,0 +1,261 @@
+"""Pins the oracle (CPU restatement of the reference propagator).
+
+1. Against the golden vectors written by the compiled reference
+   (tests/golden/, see make_golden.py): equal to <= 1e-13 (bitwise in
+   practice).
+2. Against the compiled reference itself (oracle/_ref) where it is built.
+3. Against what the mathematics fixes, independently of the reference code:
+   PMNS unitarity, matter eigenvalues vs numpy.linalg.eigvalsh, one layer vs
+   scipy.linalg.expm, the vacuum formula, chord geometry, path unitarity,
+   T-symmetry at delta = 0 and the nu / nu-bar relation.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import golden, osc_for
+from cudaprob3_amd import workloads as W
+
+TOL_GOLD = 1e-13
+NOM = [W.TH12, W.TH13, W.TH23, W.DCP, W.DM12, W.DM23]
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_prem4_golden(oracle, nutype):
+    g = golden(f"g0_prem4_{'nubar' if nutype else 'nu'}")
+    out = oracle.atmos(nutype, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                       g["yps"], osc_for(NOM, nutype), 22.0, threads=4)
+    s = int(g["stride"])
+    assert np.abs(out[::s, ::s] - g["prob"]).max() <= TOL_GOLD
+    assert np.allclose(out.sum(axis=(0, 1)), g["chan_sum"], rtol=1e-13,
+                       atol=0)
+    assert np.allclose((out * out).sum(axis=(0, 1)), g["chan_sumsq"],
+                       rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_shells12_golden(oracle, nutype):
+    g = golden(f"g1_shells12_{'nubar' if nutype else 'nu'}")
+    out = oracle.atmos(nutype, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                       g["yps"], osc_for(NOM, nutype), 22.0, threads=4)
+    assert np.abs(out - g["prob"]).max() <= TOL_GOLD
+
+
+def test_shells59_golden(oracle):
+    g = golden("g3_shells59")
+    for nt in (0, 1):
+        for i, par in enumerate(g["params"]):
+            out = oracle.atmos(nt, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                               g["yps"], osc_for(par, nt), 22.0, threads=4)
+            assert np.abs(out - g["prob"][nt, i]).max() <= TOL_GOLD
+
+
+def test_beam_golden(oracle):
+    g = golden("g2_beam")
+    for nt in (0, 1):
+        for i, par in enumerate(g["params"]):
+            out = oracle.beam(nt, g["ev"], float(g["rho"]), float(g["L_km"]),
+                              osc_for(par, nt))
+            assert np.abs(out - g["prob"][nt, i]).max() <= TOL_GOLD
+
+
+def test_edge_golden(oracle):
+    g = golden("g4_edges")
+    for key in ["nominal", "dm12_zero", "dm23_zero", "dcp_zero", "inverted"]:
+        for nt in (0, 1):
+            out = oracle.atmos(nt, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                               g["yps"], osc_for(g[f"params_{key}"], nt),
+                               22.0)
+            assert np.abs(out - g[f"prob_{key}_{nt}"]).max() <= TOL_GOLD, key
+
+
+def test_density_file_golden(oracle, tmp_path):
+    g = golden("g5_prem4_file")
+    path = tmp_path / "model.dat"
+    # the reference's 3-column file, rewritten with repeated spaces
+    with open(path, "w") as f:
+        for r, d, y in g["model_rows"]:
+            f.write(f"{float(r)!r}  {float(d)!r}   {float(y)!r}\n")
+    out = oracle.atmos_file(0, g["cosv"], g["ev"], str(path),
+                            osc_for(NOM, 0), 22.0)
+    assert np.abs(out - g["prob"]).max() <= TOL_GOLD
+
+
+def test_oracle_matches_compiled_reference(oracle):
+    from oracle.loader import load_ref
+    ref = load_ref(deep=True)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    e = W.earth_12()
+    cz, ev = W.cos_grid(40), W.log_grid(40, 0.1, 100.0)
+    for par in [NOM] + [[p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+                        for p in W.draw_param_sets(2, 99)]:
+        for nt in (0, 1):
+            a = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps,
+                             osc_for(par, nt), 22.0, threads=4)
+            b = ref.atmos(nt, cz, ev, e.radii, e.rhos, e.yps,
+                          osc_for(par, nt), 22.0, threads=4)
+            assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- math pins
+
+def pdg_u(t12, t13, t23, d):
+    s12, c12 = np.sin(t12), np.cos(t12)
+    s13, c13 = np.sin(t13), np.cos(t13)
+    s23, c23 = np.sin(t23), np.cos(t23)
+    R23 = np.array([[1, 0, 0], [0, c23, s23], [0, -s23, c23]], complex)
+    U13 = np.array([[c13, 0, s13 * np.exp(-1j * d)], [0, 1, 0],
+                    [-s13 * np.exp(1j * d), 0, c13]])
+    R12 = np.array([[c12, s12, 0], [-s12, c12, 0], [0, 0, 1]], complex)
+    return R23 @ U13 @ R12
+
+
+def test_pmns_closed_form(oracle):
+    for knu in (1, -1):
+        U = oracle.pmns(W.TH12, W.TH13, W.TH23, W.DCP, knu)
+        assert np.abs(U @ U.conj().T - np.eye(3)).max() < 1e-15
+        expect = pdg_u(W.TH12, W.TH13, W.TH23, W.DCP * knu)
+        assert np.abs(U - expect).max() < 1e-15
+
+
+def mass_matrix(osc, E, rho_ye, nutype):
+    """2E*H in the mass basis, eV^2: diag(0, dm21, dm31) + A U^+ e e^+ U."""
+    U = pdg_u(osc.th12, osc.th13, osc.th23, osc.dcp * osc.knutype)
+    d = np.diag([0.0, osc.dm12, osc.dm12 + osc.dm23])
+    A = 1.52588e-4 * E * rho_ye * (-1 if nutype else 1)
+    return d + A * np.outer(U[0].conj(), U[0])
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_matter_eigenvalues(oracle, nutype):
+    osc = osc_for(NOM, nutype)
+    for E in [0.1, 0.5, 2.0, 5.0, 20.0, 100.0]:
+        for rye in [0.0, 1.5, 6.5]:
+            lam, order = oracle.matter_eigs(osc, E, rye, nutype)
+            ev = np.linalg.eigvalsh(mass_matrix(osc, E, rye, nutype))
+            assert np.abs(np.sort(lam) - ev).max() < 1e-15 + 1e-12 * np.abs(ev).max()
+            assert sorted(order) == [0, 1, 2]
+    # continuity to vacuum: at zero density l_i = DM(i,0)
+    lam, _ = oracle.matter_eigs(osc, 1.0, 0.0, nutype)
+    assert np.abs(lam - [0.0, W.DM12, W.DM12 + W.DM23]).max() < 1e-15
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_one_layer_is_exponential(oracle, nutype):
+    """T = U expm(-i 2.534 M L / E) U^+ (flavour basis, [after][before])."""
+    osc = osc_for(NOM, nutype)
+    U = pdg_u(osc.th12, osc.th13, osc.th23, osc.dcp * osc.knutype)
+    for E, rye, L in [(0.3, 1.6, 500.0), (3.0, 6.5, 2400.0),
+                      (25.0, 4.0, 9000.0), (1.0, 0.0, 295.0)]:
+        T, P = oracle.layer(osc, E, rye, L, nutype)
+        M = mass_matrix(osc, E, rye, nutype)
+        expect = U @ scipy.linalg.expm(-1j * 2.534 * M * L / E) @ U.conj().T
+        assert np.abs(T - expect).max() < 1e-12
+        # Sylvester projectors: idempotent, complete, eigen-projectors of M
+        Pk = [P[:, :, k] for k in range(3)]
+        assert np.abs(sum(Pk) - np.eye(3)).max() < 1e-11
+        for p in Pk:
+            assert np.abs(p @ p - p).max() < 1e-11
+
+
+def vacuum_prob(osc, L, E):
+    U = pdg_u(osc.th12, osc.th13, osc.th23, osc.dcp * osc.knutype)
+    m = np.array([0.0, osc.dm12, osc.dm12 + osc.dm23])
+    amp = U @ np.diag(np.exp(-1j * 2.534 * m * L / E)) @ U.conj().T
+    return np.abs(amp) ** 2  # [after][before]
+
+
+def test_vacuum_paths(oracle):
+    """cos >= 0 crosses no shell: P(a->b) = |sum_k U_bk U*_ak e^{-i..}|^2
+    over the path length from the production height."""
+    e = W.prem4()
+    osc = osc_for(NOM, 0)
+    cz = np.array([0.0, 0.3, 0.9, 1.0])
+    ev = np.array([0.2, 1.0, 7.0])
+    out = oracle.atmos(0, cz, ev, e.radii, e.rhos, e.yps, osc, 15.0)
+    R, h = 6371.0, 15.0
+    for i, c in enumerate(cz):
+        L = np.sqrt((R + h) ** 2 - R * R * (1 - c * c)) - R * c
+        for j, E in enumerate(ev):
+            P = vacuum_prob(osc, L, E)
+            got = out[i, j].reshape(3, 3)  # [a][b] = P(a->b)
+            assert np.abs(got - P.T).max() < 1e-12
+
+
+def test_beam_vacuum_limit(oracle):
+    osc = osc_for(NOM, 0)
+    ev = np.array([0.3, 1.0, 2.5])
+    out = oracle.beam(0, ev, 0.0, 810.0, osc)
+    for j, E in enumerate(ev):
+        assert np.abs(out[j].reshape(3, 3) - vacuum_prob(osc, 810.0, E).T).max() < 1e-12
+
+
+def test_geometry(oracle):
+    """Segments sum to the path length; shells are crossed symmetrically;
+    each shell's chord is 2 sqrt(r^2 - b^2)."""
+    e = W.earth_12()
+    R, h = 6371.0, 22.0
+    for c in [-1.0, -0.93, -0.5, -0.21, -0.04]:
+        M, seg, rye = oracle.segments(c, e.radii, e.rhos, e.yps, h)
+        L = np.sqrt((R + h) ** 2 - R * R * (1 - c * c)) - R * c
+        total = seg[0] + 2 * seg[1:M].sum() + seg[M]
+        assert abs(total - L) < 1e-9
+        b = R * np.sqrt(1 - c * c)
+        radii = e.radii[::-1]
+        chords = 2 * np.sqrt(np.maximum(radii[:M] ** 2 - b * b, 0))
+        # the innermost shell is crossed once along its full chord
+        assert abs(seg[M] - chords[M - 1]) < 1e-9
+        assert abs(seg[0] - (L - (-2 * c * R))) < 1e-9
+        expect_M = int(np.sum(c < -np.sqrt(1 - (radii[1:] / R) ** 2))) + 1
+        assert M == expect_M
+
+
+def test_path_invariants(oracle):
+    """Unitarity of P; T-symmetry P(a->b) = P(b->a) at delta = 0 on the
+    symmetric Earth path (no atmosphere); nu-bar equals nu with
+    (delta, A) -> (-delta, -A)."""
+    e = W.earth_12()
+    cz = np.array([-0.999, -0.8, -0.45, -0.1])
+    ev = W.log_grid(15, 0.2, 80.0)
+    p = osc_for(NOM, 0)
+    out = oracle.atmos(0, cz, ev, e.radii, e.rhos, e.yps, p, 22.0)
+    P = out.reshape(-1, 3, 3)
+    assert np.abs(P.sum(-1) - 1).max() < 1e-12
+    assert np.abs(P.sum(-2) - 1).max() < 1e-12
+    # T-symmetry: h = 0 removes the atmosphere segment, so the path is
+    # palindromic; with delta = 0 the amplitude matrix is symmetric.
+    p0 = osc_for([W.TH12, W.TH13, W.TH23, 0.0, W.DM12, W.DM23], 0)
+    out0 = oracle.atmos(0, cz, ev, e.radii, e.rhos, e.yps, p0, 0.0)
+    P0 = out0.reshape(-1, 3, 3)
+    assert np.abs(P0 - P0.transpose(0, 2, 1)).max() < 1e-12
+    # antineutrino relation: calculate(Antineutrino) with kNuType = -1 equals
+    # the neutrino code path run with the conjugated parameters, checked
+    # through T-symmetry of the conjugate: P_nubar(a->b; d) = P_nu(b->a; -d)
+    # only holds for symmetric paths; use h = 0 again.
+    pb = osc_for(NOM, 1)
+    outb = oracle.atmos(1, cz, ev, e.radii, e.rhos, e.yps, pb, 0.0)
+    Pb = outb.reshape(-1, 3, 3)
+    assert np.abs(Pb.sum(-1) - 1).max() < 1e-12
+    # CPT-free check of the sign conventions: at zero density nu-bar with
+    # kNuType = -1 is the complex conjugate amplitude, P_nubar(a->b) =
+    # P_nu(b->a)
+    z = np.zeros_like(e.rhos)
+    pn = oracle.atmos(0, cz, ev, e.radii, z, e.yps, p, 0.0).reshape(-1, 3, 3)
+    pa = oracle.atmos(1, cz, ev, e.radii, z, e.yps, pb, 0.0).reshape(-1, 3, 3)
+    assert np.abs(pa - pn.transpose(0, 2, 1)).max() < 1e-12
+
+
+def test_oracle_errors(oracle):
+    e = W.prem4()
+    with pytest.raises(RuntimeError, match="radii order"):
+        oracle.atmos(0, [-0.5], [1.0], [0, 3000, 1000, 6371], [1, 2, 3, 4],
+                     [0.5] * 4, osc_for(NOM, 0), 22.0)
+    with pytest.raises(RuntimeError, match="must not be empty"):
+        oracle.atmos(0, [-0.5], [1.0], [], [], [], osc_for(NOM, 0), 22.0)
+    with pytest.raises(RuntimeError, match="could not open"):
+        oracle.atmos_file(0, [-0.5], [1.0], "/nonexistent.dat",
+                          osc_for(NOM, 0), 22.0)
+    del e
