@@ -1,0 +1,160 @@
+// Drives HipPropagatorSingle<double> / HipBeamPropagatorSingle<double> the way
+// example/mainatmos.cpp:116-160 and example/mainbeam.cpp:112-139 drive the
+// reference GPU propagators, and writes every probability to a binary file
+// for the parity test (tests/test_gpu_dropin.py).
+//
+//   dropin_main atmos <out.bin> <precision 0|1|2> <nutype 0|1>
+//   dropin_main beam  <out.bin> <precision 0|1|2> <nutype 0|1>
+//   dropin_main errors
+//
+// atmos: the 4-shell model, 100 x 100 grid (cos -1 + i/100, E 0.1-100 GeV
+// log), nominal parameters, h = 22 km; output [cos][E][9] doubles.
+// beam: 1300 km, 2.6 g/cm^3, 500 energies over 0.1-20 GeV; output [E][9].
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cudaprob3_amd/hip_propagator.hpp"
+
+using namespace cp3hip;
+
+static std::vector<double> logspace(double lo, double hi, int n) {
+  std::vector<double> v((size_t)n);
+  for (int i = 0; i < n; ++i) v[(size_t)i] = lo * std::pow(hi / lo, (double)i / (n - 1));
+  return v;
+}
+
+static const double kTh12 = std::asin(std::sqrt(0.307));
+static const double kTh13 = std::asin(std::sqrt(0.0220));
+static const double kTh23 = std::asin(std::sqrt(0.546));
+static const double kDcp = -1.6, kDm12 = 7.53e-5, kDm23 = 2.494e-3;
+
+static int run_atmos(const char* path, int precision, int nutype) {
+  const int nC = 100, nE = 100;
+  std::vector<double> cz((size_t)nC);
+  for (int i = 0; i < nC; ++i) cz[(size_t)i] = -1.0 + i / 100.0;
+  const std::vector<double> ev = logspace(0.1, 100.0, nE);
+  HipPropagatorSingle<double> p(0, nC, nE, (Precision)precision);
+  p.setEnergyList(ev);
+  p.setCosineList(cz);
+  p.setMNSMatrix(kTh12, kTh13, kTh23, kDcp, nutype ? -1 : 1);
+  p.setNeutrinoMasses(kDm12, kDm23);
+  p.setDensity({0.0, 1220.0, 3480.0, 5701.0, 6371.0}, {13.0, 13.0, 11.3, 5.0, 3.3},
+               {0.5, 0.5, 0.5, 0.5, 0.5});
+  p.setProductionHeight(22.0);
+  p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
+  std::vector<double> out((size_t)nC * nE * 9);
+  for (int c = 0; c < nC; ++c)
+    for (int e = 0; e < nE; ++e)
+      for (int t = 0; t < 9; ++t)
+        out[((size_t)c * nE + e) * 9 + t] = p.getProbability(c, e, (ProbType)t);
+  // getProbabilityArr: energy outer, cosine inner
+  std::vector<double> arr((size_t)nC * nE);
+  p.getProbabilityArr(arr.data(), m_e);
+  for (int e = 0; e < nE; ++e)
+    for (int c = 0; c < nC; ++c)
+      if (arr[(size_t)e * nC + c] != out[((size_t)c * nE + e) * 9 + m_e]) {
+        std::fprintf(stderr, "getProbabilityArr order mismatch\n");
+        return 3;
+      }
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 2;
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+  std::printf("atmos ok: %d x %d, kernel %.3f ms\n", nC, nE, p.lastKernelMs());
+  return 0;
+}
+
+static int run_beam(const char* path, int precision, int nutype) {
+  const int nE = 500;
+  const std::vector<double> ev = logspace(0.1, 20.0, nE);
+  HipBeamPropagatorSingle<double> p(0, nE, 1, (Precision)precision);
+  p.setEnergyList(ev);
+  p.setMNSMatrix(kTh12, kTh13, kTh23, kDcp, nutype ? -1 : 1);
+  p.setNeutrinoMasses(kDm12, kDm23);
+  p.setDensity(2.6);
+  p.setPathLength(1300.0);
+  p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
+  std::vector<double> out((size_t)nE * 9);
+  for (int e = 0; e < nE; ++e)
+    for (int t = 0; t < 9; ++t) out[(size_t)e * 9 + t] = p.getProbability(e, (ProbType)t);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 2;
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+  std::printf("beam ok: %d energies\n", nE);
+  return 0;
+}
+
+template <class Fn>
+static bool throws(Fn fn, const char* needle) {
+  try {
+    fn();
+  } catch (const std::runtime_error& e) {
+    if (std::strstr(e.what(), needle)) return true;
+    std::fprintf(stderr, "unexpected message: %s\n", e.what());
+    return false;
+  }
+  return false;
+}
+
+// The reference's exceptions (atmoscpupropagator.hpp:123-142, 369-370,
+// 387-388; propagator.hpp:214-215; atmoscudapropagator.cu:223-224, 267-268).
+static int run_errors() {
+  int bad = 0;
+  HipPropagatorSingle<double> p(0, 3, 4);
+  bad += !throws([&] { p.setEnergyList({1.0, 2.0}); }, "number of energy nodes");
+  bad += !throws([&] { p.setCosineList({-1.0}); }, "number of cosine nodes");
+  bad += !throws([&] { p.setProductionHeight(22.0); }, "must set cosine list");
+  bad += !throws([&] { p.setDensity({0.0, 3000.0, 1000.0}, {1, 2, 3}, {0.5, 0.5, 0.5}); },
+                 "radii order");
+  bad += !throws([&] { p.setDensity({0.0, 1.0}, {1.0}, {0.5, 0.5}); }, "rhos.size() != radii.size()");
+  bad += !throws([&] { p.setDensity(std::vector<double>{}, {}, {}); }, "must not be empty");
+  bad += !throws([&] { p.setDensityFromFile("/nonexistent/model.dat"); }, "could not open");
+  p.setCosineList({-1.0, -0.5, 0.2});
+  p.setEnergyList({1.0, 2.0, 3.0, 4.0});
+  bad += !throws([&] { p.calculateProbabilities(Neutrino); }, "production height was not set");
+  p.setMNSMatrix(kTh12, kTh13, kTh23, kDcp, 1);
+  p.setNeutrinoMasses(kDm12, kDm23);
+  p.setDensity({0.0, 3480.0, 6371.0}, {11.3, 11.3, 4.0}, {0.5, 0.5, 0.5});
+  p.setProductionHeight(15.0);
+  p.calculateProbabilities(Neutrino);
+  bad += !throws([&] { p.getProbability(3, 0, e_e); }, "Invalid indices");
+  bad += !throws([&] { p.getProbability(0, 4, e_e); }, "Invalid indices");
+  bad += !throws([&] { p.setChemicalComposition({0.5}); }, "Size of input list");
+  bad += !throws([&] { p.setProductionHeightList({1.0}, {1.0, 2.0}); }, "not expecting");
+  HipPropagatorSingle<double> q(std::move(p));
+  bad += !throws([&] { p.calculateProbabilities(Neutrino); }, "moved from");
+  q.calculateProbabilities(Antineutrino);
+  double s = 0;
+  for (int t = 0; t < 3; ++t) s += q.getProbability(0, 0, (ProbType)t);
+  if (std::fabs(s - 1.0) > 1e-12) {
+    std::fprintf(stderr, "unitarity %.17g\n", s);
+    ++bad;
+  }
+  bad += !throws([] { HipPropagatorSingle<double> r(1 << 20, 2, 2); }, "not available");
+  std::printf("errors: %d unexpected\n", bad);
+  return bad ? 1 : 0;
+}
+
+int main(int argc, char** argv) {
+  try {
+    if (argc >= 2 && std::string(argv[1]) == "errors") return run_errors();
+    if (argc < 5) {
+      std::fprintf(stderr, "usage: %s atmos|beam out.bin precision nutype | errors\n", argv[0]);
+      return 2;
+    }
+    const std::string mode = argv[1];
+    const int prec = std::atoi(argv[3]), nt = std::atoi(argv[4]);
+    if (mode == "atmos") return run_atmos(argv[2], prec, nt);
+    if (mode == "beam") return run_beam(argv[2], prec, nt);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "exception: %s\n", e.what());
+    return 4;
+  }
+  return 2;
+}

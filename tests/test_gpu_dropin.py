@@ -1,0 +1,64 @@
+"""The C++ drop-in classes (include/cudaprob3_amd/hip_propagator.hpp), driven
+by tests/cpp/dropin_main.cpp the way the reference examples drive
+AtmosCudaPropagatorSingle / BeamCudaPropagatorSingle, against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, osc_for
+from cudaprob3_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "tests", "cpp", "build", "dropin_main")
+NOM = [W.TH12, W.TH13, W.TH23, W.DCP, W.DM12, W.DM23]
+
+
+def run(*args):
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (run __graft_entry__.build())")
+    r = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    return r
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_dropin_atmos_fp64(oracle, tmp_path, nutype):
+    out = tmp_path / "a.bin"
+    r = run("atmos", out, 0, nutype)
+    assert r.returncode == 0
+    got = np.fromfile(out, dtype=np.float64).reshape(100, 100, 9)
+    e = W.prem4()
+    ref = oracle.atmos(nutype, W.cos_grid(100), W.log_grid(100, 0.1, 100.0),
+                       e.radii, e.rhos, e.yps, osc_for(NOM, nutype), 22.0,
+                       threads=4)
+    assert np.abs(got - ref).max() < 1e-10
+
+
+def test_dropin_atmos_mixed(oracle, tmp_path):
+    out = tmp_path / "a.bin"
+    assert run("atmos", out, 2, 0).returncode == 0
+    got = np.fromfile(out, dtype=np.float64).reshape(100, 100, 9)
+    e = W.prem4()
+    ref = oracle.atmos(0, W.cos_grid(100), W.log_grid(100, 0.1, 100.0),
+                       e.radii, e.rhos, e.yps, osc_for(NOM, 0), 22.0,
+                       threads=4)
+    assert np.abs(got - ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_dropin_beam_fp64(oracle, tmp_path, nutype):
+    out = tmp_path / "b.bin"
+    assert run("beam", out, 0, nutype).returncode == 0
+    got = np.fromfile(out, dtype=np.float64).reshape(500, 9)
+    ref = oracle.beam(nutype, W.log_grid(500, 0.1, 20.0), 2.6, 1300.0,
+                      osc_for(NOM, nutype))
+    assert np.abs(got - ref).max() < 1e-10
+
+
+def test_dropin_errors():
+    r = run("errors")
+    assert r.returncode == 0, r.stderr
