@@ -1,0 +1,230 @@
+"""Benchmark: probability evaluations per second of the batched propagator on
+one MI355X.
+
+Workload (default): 1024 seeded parameter sets on the 59-shell seeded Earth
+model, half on 400 x 400 and half on 200 x 200 (cos, E) grids, E 0.5-50 GeV,
+each for neutrinos and antineutrinos: 2048 jobs, 2.048e8 evaluations per step,
+in one persistent launch replayed from a hipGraph.  A step is one pass over
+the whole batch with every input resident in HBM.  The mixed form (FP64
+eigen-solve and phases, FP32 amplitudes) is the headline; the plain FP64 form
+of the same batch is timed beside it.
+
+Prints one JSON line (see DESIGN.md for the fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from cudaprob3_amd import workloads as W  # noqa: E402
+from cudaprob3_amd.batch import ATMOS, Batch, make_job  # noqa: E402
+from cudaprob3_amd import flops as FL  # noqa: E402
+
+METRIC = "prob evals/s (grid pts x param sets), p50/p99 latency; mixed vs plain FP64"
+PEAK_F64 = 78.6e12   # MI355X FP64 vector/matrix, FLOP/s
+PEAK_F32 = 157.3e12  # MI355X FP32 vector (published, packed), FLOP/s
+PEAK_HBM = 8.0e12    # B/s
+
+
+def build_batch(precision, wl, tuning=None, graph=True):
+    b = Batch(precision)
+    grids = {}
+    for j in wl.jobs:
+        if j.grid not in grids:
+            grids[j.grid] = b.add_atmos_grid(j.cosv, j.ev, wl.earth, wl.h_km)
+    b.set_jobs([make_job(ATMOS, grids[j.grid], j.params, j.nutype)
+                for j in wl.jobs])
+    if tuning is None:
+        tuning = b.autotune(reps=2)
+    else:
+        b.set_tuning(*tuning)
+    b.use_graph(graph)
+    return b, tuning
+
+
+def time_steps(b, steps, warmup):
+    for _ in range(warmup):
+        b.run()
+    b.wait()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+        b.wait()
+        kernel_ms.append(b.last_ms())
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps, np.array(kernel_ms)
+
+
+def cpu_baseline(wl, gpu_batch, budget_s=12.0):
+    """The reference CPU propagator (compiled from the reference headers into
+    oracle/_ref, or the oracle restatement where that was not built), timed on
+    a bounded sample of the same jobs with all host cores given to it; the
+    GPU results of the same jobs are compared to its output."""
+    sys.path.insert(0, ROOT)
+    from oracle.loader import Osc, load_oracle, load_ref
+    lib = load_ref(deep=True)
+    kind = "reference"
+    if lib is None:
+        lib, kind = load_oracle(), "port"
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    order = []  # alternate big/small grids and nu/nubar
+    half = len(wl.jobs) // 2
+    for i in range(0, half, 2):
+        order += [i, i + 1, half + i, half + i + 1]
+    evals, elapsed, worst, sq, npts, used = 0, 0.0, 0.0, 0.0, 0, 0
+    for j in order:
+        job = wl.jobs[j]
+        t12, t13, t23, d, k, m12, m23 = job.params.for_nutype(job.nutype)
+        osc = Osc(t12, t13, t23, d, m12, m23, k)
+        t0 = time.perf_counter()
+        ref = lib.atmos(job.nutype, job.cosv, job.ev, wl.earth.radii,
+                        wl.earth.rhos, wl.earth.yps, osc, wl.h_km,
+                        threads=cores)
+        elapsed += time.perf_counter() - t0
+        evals += ref.shape[0] * ref.shape[1]
+        gpu = gpu_batch.result(j)  # [9][nC][nE]
+        diff = np.abs(np.transpose(gpu, (1, 2, 0)) - ref)
+        worst = max(worst, float(diff.max()))
+        sq += float((diff ** 2).sum())
+        npts += diff.size
+        used += 1
+        if elapsed >= budget_s:
+            break
+    return {
+        "value": evals / elapsed, "unit": "evals/s", "cores": cores,
+        "kind": kind,
+        "sample": f"{used} of {len(wl.jobs)} jobs ({evals} evals, "
+                  f"{elapsed:.1f} s, AtmosCpuPropagator<double> with OpenMP)",
+    }, worst, float(np.sqrt(sq / max(npts, 1)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--sets", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--tune", default="auto",
+                    help="'auto' (autotune) or 'NC,WAVES' for both forms")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the propagator shards by parameter set with no exchange; "
+              "see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    wl = W.atmos1024(n_sets=args.sets)
+    evals = wl.n_evals
+    graph = not args.no_graph
+
+    fixed = None
+    if args.tune != "auto":
+        nc_s, w_s = args.tune.split(",")
+        fixed = (int(nc_s), int(w_s))
+    mixed, tune_m = build_batch("mixed", wl, tuning=fixed, graph=graph)
+    ms_step, kms = time_steps(mixed, args.steps, args.warmup)
+    value = evals / (ms_step / 1e3)
+
+    counts = FL.batch_counts(wl, "mixed", tune_m[0])
+    t_k = float(np.mean(kms)) / 1e3
+    achieved = (counts["f64"] + counts["f32"] * PEAK_F64 / PEAK_F32) / t_k
+    out_bytes = evals * 9 * 4
+    traffic = FL.measured_traffic("atmos_batch_mixed")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 eigen-solve/phases + f32 amplitudes (mixed)",
+        "data": "synthetic (seeded Earth model and parameter sets)",
+        "config": {
+            "workload": f"atmos: {args.sets} param sets x (nu, nubar), 59-shell "
+                        "seeded Earth, half 400x400 half 200x200 (cos x E), "
+                        "E 0.5-50 GeV, h 22 km; one persistent "
+                        + ("hipGraph-replayed " if graph else "") + "launch",
+            "jobs": len(wl.jobs), "evals_per_step": evals,
+            "precision": "mixed",
+            "tuning": {"cosine_block": tune_m[0], "waves_per_cu": tune_m[1]},
+            "parallelism": "1 GPU",
+        },
+        "kernel_ms": {"mean": float(np.mean(kms)),
+                      "p50": float(np.percentile(kms, 50)),
+                      "p99": float(np.percentile(kms, 99))},
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": PEAK_F64 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_F64,
+            "traffic": traffic,
+            "note": "FP64-equivalent rate (F64 + F32*78.6/157.3)/t_kernel of "
+                    "the counted FLOPs of the exact kernel decomposition; "
+                    "transcendentals counted separately",
+            "flops_f64": counts["f64"], "flops_f32": counts["f32"],
+            "sincos_f64": counts["sincos64"], "sincos_f32": counts["sincos32"],
+            "acos_f64": counts["acos64"],
+            "output_GBps": out_bytes / t_k / 1e9,
+            "output_frac_of_hbm": out_bytes / t_k / PEAK_HBM,
+        },
+    }
+    mixed_results = mixed
+
+    if not args.no_fp64:
+        fp64, tune_d = build_batch("fp64", wl, tuning=fixed, graph=graph)
+        ms64, kms64 = time_steps(fp64, max(3, args.steps // 2), 1)
+        v64 = evals / (ms64 / 1e3)
+        c64 = FL.batch_counts(wl, "fp64", tune_d[0])
+        t64 = float(np.mean(kms64)) / 1e3
+        # mixed vs FP64 on the GPU, over sampled jobs
+        worst, sq, n = 0.0, 0.0, 0
+        for j in range(0, len(wl.jobs), max(1, len(wl.jobs) // 16)):
+            a, d = mixed.result(j), fp64.result(j)
+            diff = np.abs(a - d)
+            worst = max(worst, float(diff.max()))
+            sq += float((diff ** 2).sum())
+            n += diff.size
+        result["fp64"] = {
+            "value": v64, "ms_per_step": ms64,
+            "kernel_ms": float(np.mean(kms64)),
+            "tuning": {"cosine_block": tune_d[0], "waves_per_cu": tune_d[1]},
+            "frac_of_fp64_peak": c64["f64"] / t64 / PEAK_F64,
+        }
+        result["mixed_over_fp64"] = value / v64
+        result["mixed_err_vs_fp64_gpu"] = {"max": worst,
+                                           "rms": float(np.sqrt(sq / n))}
+        fp64.close()
+
+    if not args.no_cpu:
+        cb, worst, rms = cpu_baseline(wl, mixed_results)
+        result["cpu_baseline"] = cb
+        result["mixed_err_vs_cpu_reference"] = {"max": worst, "rms": rms}
+        result["gpu_over_cpu"] = value / cb["value"]
+    mixed.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -80,6 +80,36 @@ _SIGS = {
     "cp3_beam_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
 }
 
+
+class Cp3Job(C.Structure):
+    """struct cp3_job (include/cp3_hip.h)."""
+    _fields_ = [("kind", C.c_int32), ("grid", C.c_int32),
+                ("nutype", C.c_int32), ("knutype", C.c_int32),
+                ("theta12", C.c_double), ("theta13", C.c_double),
+                ("theta23", C.c_double), ("dcp", C.c_double),
+                ("dm12sq", C.c_double), ("dm23sq", C.c_double)]
+
+
+_i64p = C.POINTER(C.c_int64)
+_SIGS.update({
+    "cp3_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
+    "cp3_batch_destroy": (C.c_int, [_vp]),
+    "cp3_batch_add_atmos_grid": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, _ip]),
+    "cp3_batch_add_beam_grid": (C.c_int, [_vp, _dp, C.c_int, C.c_double, C.c_double, _ip]),
+    "cp3_batch_set_jobs": (C.c_int, [_vp, C.POINTER(Cp3Job), C.c_int]),
+    "cp3_batch_update_jobs": (C.c_int, [_vp, C.POINTER(Cp3Job), C.c_int]),
+    "cp3_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "cp3_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "cp3_batch_use_graph": (C.c_int, [_vp, C.c_int]),
+    "cp3_batch_run": (C.c_int, [_vp]),
+    "cp3_batch_wait": (C.c_int, [_vp]),
+    "cp3_batch_last_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "cp3_batch_result_device": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), _ip]),
+    "cp3_batch_copy_result": (C.c_int, [_vp, C.c_int, _dp]),
+    "cp3_batch_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p]),
+    "cp3_batch_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+})
+
 _lib = None
 
 

@@ -1,0 +1,562 @@
+// Batched engine (include/cp3_hip.h, cp3_batch_*): resident grids, a job list
+// of (parameter set, neutrino type, grid), one persistent launch per kind over
+// the flattened work list of all jobs, optional hipGraph replay, and an
+// autotuner for the cosine block and the persistent grid size.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cp3_hip.h"
+#include "cp3_capi_util.hpp"
+#include "cp3_host.hpp"
+#include "cp3_launch.hpp"
+
+using cp3::capi::DevBuf;
+using cp3::capi::DeviceScope;
+using cp3::capi::Error;
+using cp3::capi::guard;
+using cp3::capi::hip_check;
+using cp3::capi::PinnedBuf;
+
+namespace {
+
+// One resident atmospheric grid (trajectory table on the device).
+struct AtmosGrid {
+  int nC = 0, nE = 0, S = 0;
+  DevBuf ev, perm, M, Lvac, seg, rye;
+  cp3::GridDev dev() const {
+    cp3::GridDev g{};
+    g.ev = (const double*)ev.p;
+    g.perm = (const int32_t*)perm.p;
+    g.M = (const int32_t*)M.p;
+    g.Lvac = (const double*)Lvac.p;
+    g.seg = (const double*)seg.p;
+    g.rye = (const double*)rye.p;
+    g.nC = nC;
+    g.nE = nE;
+    g.S = S;
+    return g;
+  }
+};
+
+struct BeamGrid {
+  int nE = 0;
+  double rye = 0.0, L = 0.0;
+  DevBuf ev;
+};
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+}  // namespace
+
+struct cp3_batch {
+  int dev = 0, precision = CP3_FP64;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ran = false;
+  std::vector<std::unique_ptr<AtmosGrid>> agrids;
+  std::vector<std::unique_ptr<BeamGrid>> bgrids;
+
+  std::vector<cp3_job> jobs;
+  std::vector<int> slot;  // job -> index in its kind's array
+  std::vector<cp3::AtmosJob> ajobs;
+  std::vector<cp3::BeamJob> bjobs;
+  DevBuf d_ajobs, d_bjobs, d_out;
+  PinnedBuf staging;
+  std::vector<size_t> out_off, out_len;
+  int64_t a_units = 0, b_units = 0, evals = 0;
+  size_t out_total = 0;
+
+  int tune_nc = 0, tune_waves = 0;  // 0 = automatic
+  int nc = 1, a_grid = 0, b_grid = 0;
+
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~cp3_batch() {
+    DeviceScope ds(dev);
+    drop_graph();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+
+  int elem() const { return cp3::out_elem_bytes(precision); }
+
+  // Builds the device job tables from `jobs` (constants only when
+  // `params_only`) and uploads them through pinned staging on the stream.
+  void build_tables(bool params_only) {
+    const size_t na = ajobs.size(), nb = bjobs.size();
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      const cp3_job& in = jobs[j];
+      const cp3::SetConst sc =
+          cp3::make_set(in.theta12, in.theta13, in.theta23, in.dcp,
+                        in.knutype < 0 ? -1 : 1, in.dm12sq, in.dm23sq, in.nutype);
+      if (in.kind == CP3_JOB_ATMOS) {
+        cp3::AtmosJob& a = ajobs[(size_t)slot[j]];
+        a.sc = sc;
+        if (!params_only) {
+          a.g = agrids[(size_t)in.grid]->dev();
+          a.out = (char*)d_out.p + out_off[j];
+        }
+      } else {
+        cp3::BeamJob& b = bjobs[(size_t)slot[j]];
+        b.sc = sc;
+        if (!params_only) {
+          const BeamGrid& g = *bgrids[(size_t)in.grid];
+          b.ev = (const double*)g.ev.p;
+          b.nE = g.nE;
+          b.rye = g.rye;
+          b.L = g.L;
+          b.out = (char*)d_out.p + out_off[j];
+        }
+      }
+    }
+    const size_t abytes = na * sizeof(cp3::AtmosJob);
+    const size_t bbytes = nb * sizeof(cp3::BeamJob);
+    DeviceScope ds(dev);
+    // staging may still feed an earlier copy on this stream
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    staging.alloc(abytes + bbytes);
+    if (abytes) std::memcpy(staging.p, ajobs.data(), abytes);
+    if (bbytes) std::memcpy((char*)staging.p + abytes, bjobs.data(), bbytes);
+    if (!params_only) {
+      d_ajobs.alloc(abytes, dev);
+      d_bjobs.alloc(bbytes, dev);
+    }
+    if (abytes)
+      hip_check(hipMemcpyAsync(d_ajobs.p, staging.p, abytes, hipMemcpyHostToDevice, stream),
+                "job table upload");
+    if (bbytes)
+      hip_check(hipMemcpyAsync(d_bjobs.p, (char*)staging.p + abytes, bbytes,
+                               hipMemcpyHostToDevice, stream),
+                "job table upload");
+  }
+
+  void check_job(const cp3_job& j) const {
+    if (j.kind == CP3_JOB_ATMOS) {
+      if (j.grid < 0 || (size_t)j.grid >= agrids.size())
+        throw Error(CP3_ERR_INVALID, "unknown atmospheric grid id " + std::to_string(j.grid));
+    } else if (j.kind == CP3_JOB_BEAM) {
+      if (j.grid < 0 || (size_t)j.grid >= bgrids.size())
+        throw Error(CP3_ERR_INVALID, "unknown beam grid id " + std::to_string(j.grid));
+    } else {
+      throw Error(CP3_ERR_INVALID, "job kind must be CP3_JOB_ATMOS or CP3_JOB_BEAM");
+    }
+    if (j.nutype != 0 && j.nutype != 1)
+      throw Error(CP3_ERR_INVALID, "neutrino type must be 0 or 1");
+  }
+
+  void set_jobs(const cp3_job* in, int n) {
+    jobs.assign(in, in + n);
+    for (const cp3_job& j : jobs) check_job(j);
+    slot.assign(jobs.size(), 0);
+    ajobs.clear();
+    bjobs.clear();
+    out_off.assign(jobs.size(), 0);
+    out_len.assign(jobs.size(), 0);
+    size_t off = 0;
+    evals = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      size_t pts;
+      if (jobs[j].kind == CP3_JOB_ATMOS) {
+        const AtmosGrid& g = *agrids[(size_t)jobs[j].grid];
+        pts = (size_t)g.nC * (size_t)g.nE;
+        slot[j] = (int)ajobs.size();
+        ajobs.push_back(cp3::AtmosJob{});
+      } else {
+        pts = (size_t)bgrids[(size_t)jobs[j].grid]->nE;
+        slot[j] = (int)bjobs.size();
+        bjobs.push_back(cp3::BeamJob{});
+      }
+      out_off[j] = off;
+      out_len[j] = pts * 9 * (size_t)elem();
+      off = align_up(off + out_len[j]);
+      evals += (int64_t)pts;
+    }
+    out_total = off;
+    d_out.alloc(out_total, dev);
+    choose_tuning();
+    build_tables(false);
+    drop_graph();
+  }
+
+  void update_jobs(const cp3_job* in, int n) {
+    if ((size_t)n != jobs.size())
+      throw Error(CP3_ERR_INVALID, "update_jobs: job count differs from set_jobs");
+    for (int j = 0; j < n; ++j) {
+      check_job(in[j]);
+      if (in[j].kind != jobs[(size_t)j].kind || in[j].grid != jobs[(size_t)j].grid)
+        throw Error(CP3_ERR_INVALID, "update_jobs: kinds and grids must not change");
+    }
+    jobs.assign(in, in + n);
+    build_tables(true);
+  }
+
+  // Unit prefix sums for the chosen cosine block, and grid sizes.
+  void choose_tuning() {
+    nc = tune_nc ? tune_nc : auto_nc();
+    int64_t u = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      if (jobs[j].kind != CP3_JOB_ATMOS) continue;
+      cp3::AtmosJob& a = ajobs[(size_t)slot[j]];
+      a.unit_begin = u;
+      u += cp3::atmos_units(agrids[(size_t)jobs[j].grid]->dev(), nc);
+    }
+    a_units = u;
+    u = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      if (jobs[j].kind != CP3_JOB_BEAM) continue;
+      cp3::BeamJob& b = bjobs[(size_t)slot[j]];
+      b.unit_begin = u;
+      u += cp3::beam_units(bgrids[(size_t)jobs[j].grid]->nE);
+    }
+    b_units = u;
+    a_grid = grid_for(a_units, tune_waves, true);
+    b_grid = grid_for(b_units, tune_waves, false);
+  }
+
+  int auto_nc() const {
+    // the largest block that still leaves several waves per SIMD
+    const int cap = precision == CP3_FP64 ? 4 : 8;
+    for (int c = cap; c > 1; c /= 2) {
+      int64_t u = 0;
+      for (size_t j = 0; j < jobs.size(); ++j)
+        if (jobs[j].kind == CP3_JOB_ATMOS)
+          u += cp3::atmos_units(agrids[(size_t)jobs[j].grid]->dev(), c);
+      if (u >= (int64_t)cus * 4 * 8) return c;
+    }
+    return 1;
+  }
+
+  int grid_for(int64_t units, int waves, bool atmos) const {
+    if (units <= 0) return 0;
+    if (waves < 0 || units <= (int64_t)cus) return (int)std::min<int64_t>(units, 0x7fffffff);
+    int occ = 0;
+    if (atmos)
+      hip_check(cp3::atmos_batch_occupancy(precision, nc, &occ), "occupancy");
+    else
+      hip_check(cp3::beam_batch_occupancy(precision, &occ), "occupancy");
+    if (occ <= 0) occ = 1;
+    const int per_cu = waves > 0 ? std::min(waves, occ) : occ;
+    return (int)std::min<int64_t>(units, (int64_t)cus * per_cu);
+  }
+
+  void enqueue() {
+    if (a_units > 0)
+      hip_check(cp3::launch_atmos_batch(precision, nc, (const cp3::AtmosJob*)d_ajobs.p,
+                                        (int)ajobs.size(), a_units, a_grid, stream),
+                "batched atmospheric launch");
+    if (b_units > 0)
+      hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)d_bjobs.p,
+                                       (int)bjobs.size(), b_units, b_grid, stream),
+                "batched beam launch");
+  }
+
+  void run() {
+    if (jobs.empty()) throw Error(CP3_ERR_STATE, "no jobs set");
+    DeviceScope ds(dev);
+    hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+    if (use_graph) {
+      if (!exec) {
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+        try {
+          enqueue();
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0),
+                  "hipGraphInstantiate");
+      }
+      hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+    } else {
+      enqueue();
+    }
+    hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+    ran = true;
+  }
+
+  void wait() {
+    DeviceScope ds(dev);
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+
+  float last_ms() {
+    if (!ran) throw Error(CP3_ERR_STATE, "the batch was not run");
+    DeviceScope ds(dev);
+    hip_check(hipEventSynchronize(ev1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+    return ms;
+  }
+
+  void autotune(int reps, int* best_nc, int* best_waves) {
+    if (jobs.empty()) throw Error(CP3_ERR_STATE, "no jobs set");
+    if (reps < 1) reps = 1;
+    const bool had_graph = use_graph;
+    use_graph = false;
+    drop_graph();
+    float best = 1e30f;
+    int bn = 1, bw = 0;
+    const int ncs[4] = {1, 2, 4, 8};
+    const int waves[5] = {0, -1, 2, 4, 8};
+    const bool any_atmos = !ajobs.empty();
+    for (int ci = 0; ci < (any_atmos ? 4 : 1); ++ci) {
+      for (int wi = 0; wi < 5; ++wi) {
+        tune_nc = any_atmos ? ncs[ci] : 0;
+        tune_waves = waves[wi];
+        choose_tuning();
+        build_tables(false);
+        run();  // warm-up
+        wait();
+        float tot = 0.f;
+        for (int r = 0; r < reps; ++r) {
+          run();
+          tot += last_ms();
+        }
+        if (tot / reps < best) {
+          best = tot / reps;
+          bn = tune_nc;
+          bw = tune_waves;
+        }
+      }
+    }
+    tune_nc = bn;
+    tune_waves = bw;
+    choose_tuning();
+    build_tables(false);
+    use_graph = had_graph;
+    if (best_nc) *best_nc = nc;
+    if (best_waves) *best_waves = tune_waves;
+  }
+};
+
+namespace {
+
+cp3_batch* need(cp3_batch* b) {
+  if (!b) throw Error(CP3_ERR_INVALID, "null batch handle");
+  return b;
+}
+
+void need_ptr(const void* p, const char* what) {
+  if (!p) throw Error(CP3_ERR_INVALID, std::string("null pointer: ") + what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cp3_batch_create(int device, int precision, cp3_batch** out) {
+  return guard([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    if (precision != CP3_FP64 && precision != CP3_FP32 && precision != CP3_MIXED)
+      throw Error(CP3_ERR_INVALID, "unknown precision");
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+      throw Error(CP3_ERR_INVALID,
+                  "The requested GPU Id " + std::to_string(device) + " is not available.");
+    std::unique_ptr<cp3_batch> b(new cp3_batch());
+    b->dev = device;
+    b->precision = precision;
+    DeviceScope ds(device);
+    hip_check(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device),
+              "hipDeviceGetAttribute");
+    hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&b->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&b->ev1), "hipEventCreate");
+    *out = b.release();
+  });
+}
+
+int cp3_batch_destroy(cp3_batch* b) {
+  return guard([&] { delete b; });
+}
+
+int cp3_batch_add_atmos_grid(cp3_batch* b, const double* cosv, int nC,
+                             const double* ev, int nE, const double* radii,
+                             const double* rhos, const double* yps, int nrows,
+                             double h_km, int* grid_id) {
+  return guard([&] {
+    need(b);
+    need_ptr(grid_id, "grid_id");
+    if (nC <= 0 || nE <= 0) throw Error(CP3_ERR_INVALID, "grid sizes must be positive");
+    need_ptr(cosv, "cosines");
+    need_ptr(ev, "energies");
+    if (nrows < 0) throw Error(CP3_ERR_INVALID, "negative number of model rows");
+    if (nrows > 0) {
+      need_ptr(radii, "radii");
+      need_ptr(rhos, "rhos");
+      need_ptr(yps, "yps");
+    }
+    cp3::EarthModel em;
+    cp3::earth_set(em, std::vector<double>(radii, radii + nrows),
+                   std::vector<double>(rhos, rhos + nrows),
+                   std::vector<double>(yps, yps + nrows));
+    cp3::TrajTable tt;
+    cp3::build_traj_table(tt, em, cosv, nC, h_km * 100000.0);
+    std::unique_ptr<AtmosGrid> g(new AtmosGrid());
+    g->nC = nC;
+    g->nE = nE;
+    g->S = tt.S;
+    g->ev.upload(ev, (size_t)nE * sizeof(double), b->dev);
+    g->perm.upload(tt.perm.data(), tt.perm.size() * sizeof(int32_t), b->dev);
+    g->M.upload(tt.M.data(), tt.M.size() * sizeof(int32_t), b->dev);
+    g->Lvac.upload(tt.Lvac.data(), tt.Lvac.size() * sizeof(double), b->dev);
+    g->seg.upload(tt.seg.data(), tt.seg.size() * sizeof(double), b->dev);
+    g->rye.upload(tt.rye.data(), tt.rye.size() * sizeof(double), b->dev);
+    *grid_id = (int)b->agrids.size();
+    b->agrids.push_back(std::move(g));
+  });
+}
+
+int cp3_batch_add_beam_grid(cp3_batch* b, const double* ev, int nE, double rho,
+                            double L, int* grid_id) {
+  return guard([&] {
+    need(b);
+    need_ptr(grid_id, "grid_id");
+    if (nE <= 0) throw Error(CP3_ERR_INVALID, "number of energies must be positive");
+    need_ptr(ev, "energies");
+    std::unique_ptr<BeamGrid> g(new BeamGrid());
+    g->nE = nE;
+    g->rye = rho * cp3::kBeamYe;
+    g->L = L;
+    g->ev.upload(ev, (size_t)nE * sizeof(double), b->dev);
+    *grid_id = (int)b->bgrids.size();
+    b->bgrids.push_back(std::move(g));
+  });
+}
+
+int cp3_batch_set_jobs(cp3_batch* b, const cp3_job* jobs, int n) {
+  return guard([&] {
+    need(b);
+    if (n <= 0) throw Error(CP3_ERR_INVALID, "job list must not be empty");
+    need_ptr(jobs, "jobs");
+    b->set_jobs(jobs, n);
+  });
+}
+
+int cp3_batch_update_jobs(cp3_batch* b, const cp3_job* jobs, int n) {
+  return guard([&] {
+    need(b);
+    need_ptr(jobs, "jobs");
+    b->update_jobs(jobs, n);
+  });
+}
+
+int cp3_batch_set_tuning(cp3_batch* b, int nc, int waves) {
+  return guard([&] {
+    need(b);
+    if (nc != 0 && !cp3::valid_nc(nc))
+      throw Error(CP3_ERR_INVALID, "cosine block must be 0, 1, 2, 4 or 8");
+    b->tune_nc = nc;
+    b->tune_waves = waves;
+    if (!b->jobs.empty()) {
+      b->choose_tuning();
+      b->build_tables(false);
+      b->drop_graph();
+    }
+  });
+}
+
+int cp3_batch_autotune(cp3_batch* b, int reps, int* nc, int* waves) {
+  return guard([&] { need(b)->autotune(reps, nc, waves); });
+}
+
+int cp3_batch_use_graph(cp3_batch* b, int enable) {
+  return guard([&] {
+    need(b);
+    b->use_graph = enable != 0;
+    if (!b->use_graph) b->drop_graph();
+  });
+}
+
+int cp3_batch_run(cp3_batch* b) {
+  return guard([&] { need(b)->run(); });
+}
+
+int cp3_batch_wait(cp3_batch* b) {
+  return guard([&] { need(b)->wait(); });
+}
+
+int cp3_batch_last_ms(cp3_batch* b, float* ms) {
+  return guard([&] {
+    need(b);
+    need_ptr(ms, "ms");
+    *ms = b->last_ms();
+  });
+}
+
+int cp3_batch_result_device(cp3_batch* b, int job, const void** dptr, int* elem) {
+  return guard([&] {
+    need(b);
+    need_ptr(dptr, "dptr");
+    if (job < 0 || (size_t)job >= b->jobs.size())
+      throw Error(CP3_ERR_RANGE, "job index out of range");
+    *dptr = (const char*)b->d_out.p + b->out_off[(size_t)job];
+    if (elem) *elem = b->elem();
+  });
+}
+
+int cp3_batch_copy_result(cp3_batch* b, int job, double* dst) {
+  return guard([&] {
+    need(b);
+    need_ptr(dst, "dst");
+    if (job < 0 || (size_t)job >= b->jobs.size())
+      throw Error(CP3_ERR_RANGE, "job index out of range");
+    const size_t bytes = b->out_len[(size_t)job];
+    DeviceScope ds(b->dev);
+    hip_check(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (b->precision == CP3_FP64) {
+      hip_check(hipMemcpy(dst, (const char*)b->d_out.p + b->out_off[(size_t)job], bytes,
+                          hipMemcpyDeviceToHost),
+                "result copy");
+    } else {
+      std::vector<float> tmp(bytes / sizeof(float));
+      hip_check(hipMemcpy(tmp.data(), (const char*)b->d_out.p + b->out_off[(size_t)job],
+                          bytes, hipMemcpyDeviceToHost),
+                "result copy");
+      for (size_t i = 0; i < tmp.size(); ++i) dst[i] = (double)tmp[i];
+    }
+  });
+}
+
+int cp3_batch_stats(cp3_batch* b, int64_t* units, int64_t* evals, int64_t* out_bytes) {
+  return guard([&] {
+    need(b);
+    if (units) *units = b->a_units + b->b_units;
+    if (evals) *evals = b->evals;
+    if (out_bytes) *out_bytes = (int64_t)b->out_total;
+  });
+}
+
+int cp3_batch_get_stream(cp3_batch* b, void** s) {
+  return guard([&] {
+    need(b);
+    need_ptr(s, "stream");
+    *s = (void*)b->stream;
+  });
+}
+
+}  // extern "C"

@@ -238,4 +238,33 @@ hipError_t launch_beam_batch(int precision, const BeamJob* jobs, int njobs,
   return hipGetLastError();
 }
 
+template <class P>
+static hipError_t atmos_occ(int nc, int* n) {
+  switch (nc) {
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 1>, kWave, 0);
+    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 2>, kWave, 0);
+    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 4>, kWave, 0);
+    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 8>, kWave, 0);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t atmos_batch_occupancy(int precision, int nc, int* n) {
+  switch (precision) {
+    case kFP64: return atmos_occ<PolicyF64>(nc, n);
+    case kFP32: return atmos_occ<PolicyF32>(nc, n);
+    case kMixed: return atmos_occ<PolicyMixed>(nc, n);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t beam_batch_occupancy(int precision, int* n) {
+  switch (precision) {
+    case kFP64: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyF64>, kWave, 0);
+    case kFP32: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyF32>, kWave, 0);
+    case kMixed: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyMixed>, kWave, 0);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 }  // namespace cp3

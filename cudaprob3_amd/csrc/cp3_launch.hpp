@@ -70,4 +70,8 @@ hipError_t launch_beam_single(int precision, const BeamJob& job,
 hipError_t launch_beam_batch(int precision, const BeamJob* d_jobs, int njobs,
                              int64_t nunits, int grid, hipStream_t stream);
 
+// Resident 64-lane workgroups per CU for the batched kernels (occupancy).
+hipError_t atmos_batch_occupancy(int precision, int nc, int* blocks_per_cu);
+hipError_t beam_batch_occupancy(int precision, int* blocks_per_cu);
+
 }  // namespace cp3

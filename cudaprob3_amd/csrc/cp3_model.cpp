@@ -67,9 +67,205 @@ void run_beam(const cp3::SetConst& sc, const double* ev, int nE, double rho,
   }
 }
 
+// ---------------------------------------------------------------- counting
+
+// Operation counters: add/sub/mul/div/sqrt = 1 FLOP each (a fused
+// multiply-add appears as its mul and add, 2 FLOPs); transcendental calls are
+// counted separately and not as FLOPs.
+struct OpCount {
+  double f64 = 0, f32 = 0;
+  double sincos64 = 0, sincos32 = 0, acos64 = 0, acos32 = 0;
+  double sqrt64 = 0, sqrt32 = 0, rint64 = 0;
+};
+static thread_local OpCount g_ops;
+
+template <class T>
+inline void count_flop() {
+  if (sizeof(T) == 8)
+    g_ops.f64 += 1;
+  else
+    g_ops.f32 += 1;
+}
+
+template <class T>
+struct Cnt {
+  T v;
+  Cnt() = default;
+  constexpr Cnt(double x) : v((T)x) {}
+  template <class U>
+  explicit Cnt(const Cnt<U>& o) : v((T)o.v) {}
+  explicit operator double() const { return (double)v; }
+  explicit operator float() const { return (float)v; }
+  Cnt operator-() const { return Cnt((double)-v); }
+  Cnt& operator+=(const Cnt& o) { count_flop<T>(); v += o.v; return *this; }
+  Cnt& operator-=(const Cnt& o) { count_flop<T>(); v -= o.v; return *this; }
+  Cnt& operator*=(const Cnt& o) { count_flop<T>(); v *= o.v; return *this; }
+};
+template <class T> Cnt<T> mk(T x) { Cnt<T> c; c.v = x; return c; }
+template <class T> Cnt<T> operator+(Cnt<T> a, Cnt<T> b) { count_flop<T>(); return mk<T>(a.v + b.v); }
+template <class T> Cnt<T> operator-(Cnt<T> a, Cnt<T> b) { count_flop<T>(); return mk<T>(a.v - b.v); }
+template <class T> Cnt<T> operator*(Cnt<T> a, Cnt<T> b) { count_flop<T>(); return mk<T>(a.v * b.v); }
+template <class T> Cnt<T> operator/(Cnt<T> a, Cnt<T> b) { count_flop<T>(); return mk<T>(a.v / b.v); }
+template <class T> bool operator<(Cnt<T> a, Cnt<T> b) { return a.v < b.v; }
+template <class T> bool operator>(Cnt<T> a, Cnt<T> b) { return a.v > b.v; }
+template <class T> bool operator<=(Cnt<T> a, Cnt<T> b) { return a.v <= b.v; }
+template <class T> bool operator>=(Cnt<T> a, Cnt<T> b) { return a.v >= b.v; }
+
+// Counted overloads, found by argument-dependent lookup from the templates
+// of cp3_math.hpp.
+inline Cnt<double> m_sqrt(Cnt<double> x) { g_ops.sqrt64 += 1; g_ops.f64 += 1; return mk<double>(std::sqrt(x.v)); }
+inline Cnt<float> m_sqrt(Cnt<float> x) { g_ops.sqrt32 += 1; g_ops.f32 += 1; return mk<float>(std::sqrt(x.v)); }
+inline Cnt<double> m_acos(Cnt<double> x) { g_ops.acos64 += 1; return mk<double>(std::acos(x.v)); }
+inline Cnt<float> m_acos(Cnt<float> x) { g_ops.acos32 += 1; return mk<float>(std::acos(x.v)); }
+inline Cnt<double> m_fabs(Cnt<double> x) { return mk<double>(std::fabs(x.v)); }
+inline Cnt<float> m_fabs(Cnt<float> x) { return mk<float>(std::fabs(x.v)); }
+inline Cnt<double> m_rint(Cnt<double> x) { g_ops.rint64 += 1; return mk<double>(std::rint(x.v)); }
+inline void m_sincos(Cnt<double> x, Cnt<double>* s, Cnt<double>* c) {
+  g_ops.sincos64 += 1;
+  s->v = std::sin(x.v);
+  c->v = std::cos(x.v);
+}
+inline void m_sincos(Cnt<float> x, Cnt<float>* s, Cnt<float>* c) {
+  g_ops.sincos32 += 1;
+  s->v = std::sin(x.v);
+  c->v = std::cos(x.v);
+}
+
+}  // namespace
+
+namespace cp3 {
+struct CountF64 {
+  using D = Cnt<double>;
+  using F = Cnt<double>;
+};
+struct CountF32 {
+  using D = Cnt<float>;
+  using F = Cnt<float>;
+};
+struct CountMixed {
+  using D = Cnt<double>;
+  using F = Cnt<float>;
+};
+
+// The three phase rules of cp3_math.hpp with counted scalars.
+template <>
+struct Phase<CountF64> {
+  static constexpr double kOmScale = 1.0;
+  static void eval(Cnt<double> om, Cnt<double> L, Cnt<double>& acc, Cnt<double>& wr,
+                   Cnt<double>& wi) {
+    const Cnt<double> ph = om * L;
+    acc += ph;
+    Cnt<double> s, c;
+    m_sincos(ph, &s, &c);
+    wr = c - Cnt<double>(1.0);
+    wi = s;
+  }
+  static void unit(Cnt<double> acc, Cnt<double>& cr, Cnt<double>& ci) { m_sincos(acc, &ci, &cr); }
+};
+template <>
+struct Phase<CountF32> {
+  static constexpr double kOmScale = 1.0;
+  static void eval(Cnt<float> om, Cnt<float> L, Cnt<float>& acc, Cnt<float>& wr,
+                   Cnt<float>& wi) {
+    const Cnt<float> ph = om * L;
+    acc += ph;
+    Cnt<float> s, c;
+    m_sincos(ph, &s, &c);
+    wr = c - Cnt<float>(1.0);
+    wi = s;
+  }
+  static void unit(Cnt<float> acc, Cnt<float>& cr, Cnt<float>& ci) { m_sincos(acc, &ci, &cr); }
+};
+template <>
+struct Phase<CountMixed> {
+  static constexpr double kOmScale = kInvTwoPi;
+  static void eval(Cnt<double> om, Cnt<double> L, Cnt<double>& acc, Cnt<float>& wr,
+                   Cnt<float>& wi) {
+    const Cnt<double> rev = om * L;
+    const Cnt<double> red = rev - m_rint(rev);
+    acc += red;
+    Cnt<float> s, c;
+    m_sincos(Cnt<float>(red), &s, &c);
+    wr = c - Cnt<float>(1.0);
+    wi = s;
+  }
+  static void unit(Cnt<double> acc, Cnt<float>& cr, Cnt<float>& ci) {
+    const Cnt<double> red = acc - m_rint(acc);
+    m_sincos(Cnt<float>(red), &ci, &cr);
+  }
+};
+}  // namespace cp3
+
+namespace {
+
+// Counts one call of each kernel stage; out[stage*9 + k] with stages
+// 0 solve_slab, 1 traj_middle, 2 traj_inner, 3 traj_finish (with traj_init),
+// 4 beam_point, and k = f64, f32, sincos64, sincos32, acos64, acos32,
+// sqrt64, sqrt32, rint64.
+template <class P>
+void stage_ops(double* out) {
+  using D = typename P::D;
+  using F = typename P::F;
+  const cp3::SetConst sc = cp3::make_set(0.58725, 0.14887, 0.83146, -1.6, 1, 7.53e-5,
+                                         2.494e-3, 0);
+  auto take = [&](int stage) {
+    const OpCount& o = g_ops;
+    const double v[9] = {o.f64, o.f32, o.sincos64, o.sincos32, o.acos64,
+                         o.acos32, o.sqrt64, o.sqrt32, o.rint64};
+    for (int k = 0; k < 9; ++k) out[stage * 9 + k] = v[k];
+    g_ops = OpCount();
+  };
+  g_ops = OpCount();
+  const D E = D(2.0), inv = D(0.5);
+  cp3::Slab<P> s;
+  cp3::solve_slab<P>(sc, E, inv, D(4.0 * 0.494), s);
+  take(0);
+  cp3::Traj<P> t;
+  cp3::traj_init<P>(t, false);
+  g_ops = OpCount();
+  cp3::traj_middle<P>(t, s, D(700.0));
+  take(1);
+  cp3::traj_inner<P>(t, s, D(1500.0));
+  take(2);
+  cp3::Slab<P> vac;
+  cp3::solve_slab<P>(sc, E, inv, D(0.0), vac);
+  g_ops = OpCount();
+  F prob[9];
+  cp3::traj_finish<P>(t, vac, D(80.0), sc, prob);
+  take(3);
+  cp3::beam_point<P>(sc, E, D(1.3), D(1300.0), prob);
+  take(4);
+}
+
 }  // namespace
 
 extern "C" {
+
+// Operation counts of the kernel stages (see stage_ops), 45 doubles.
+void cp3m_stage_ops(int precision, double* out) {
+  if (precision == 0)
+    stage_ops<cp3::CountF64>(out);
+  else if (precision == 1)
+    stage_ops<cp3::CountF32>(out);
+  else
+    stage_ops<cp3::CountMixed>(out);
+}
+
+// Shell counts per cosine (the M of the trajectory table) for a model.
+int cp3m_max_layers(const double* cosv, int nC, const double* radii,
+                    const double* rhos, const double* yps, int nrows,
+                    int32_t* M_out) {
+  try {
+    cp3::EarthModel em;
+    cp3::earth_set(em, std::vector<double>(radii, radii + nrows),
+                   std::vector<double>(rhos, rhos + nrows),
+                   std::vector<double>(yps, yps + nrows));
+    for (int c = 0; c < nC; ++c) M_out[c] = cp3::earth_max_layer(em, cosv[c]);
+  } catch (...) {
+    return -1;
+  }
+  return 0;
+}
 
 // Atmospheric probabilities [cos][E][9] through the kernel decomposition.
 // precision: 0 = FP64, 1 = FP32, 2 = mixed.

@@ -1,0 +1,118 @@
+"""Algorithmic work of a launch: FLOPs (split FP64 / FP32) and transcendental
+calls of the exact decomposition the kernels execute.
+
+Per-stage counts come from the host build of the kernel arithmetic run with an
+operation-counting scalar type (``cp3m_stage_ops`` in libcp3model.so):
+add/sub/mul/div/sqrt count 1, a fused multiply-add 2; sincos/acos are counted
+apart.  A launch's total follows the kernel's work list: per cosine block of
+``nc`` trajectories and per valid energy lane, one eigen-solve per shell up to
+the block's deepest shell plus one for the atmosphere; per trajectory, M-1
+middle shells, one innermost shell (M >= 1) and the finish.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+
+import numpy as np
+
+from ._native import MODEL_SO
+
+STAGES = ["solve", "middle", "inner", "finish", "beam"]
+KEYS = ["f64", "f32", "sincos64", "sincos32", "acos64", "acos32", "sqrt64",
+        "sqrt32", "rint64"]
+PREC = {"fp64": 0, "fp32": 1, "mixed": 2}
+PROFILES = os.path.join(os.path.dirname(os.path.dirname(
+    os.path.abspath(__file__))), "profiles")
+
+_model = None
+
+
+def _lib():
+    global _model
+    if _model is None:
+        if not os.path.exists(MODEL_SO):
+            raise ImportError(f"{MODEL_SO} is not built")
+        m = C.CDLL(MODEL_SO)
+        m.cp3m_stage_ops.argtypes = [C.c_int, C.POINTER(C.c_double)]
+        m.cp3m_stage_ops.restype = None
+        m.cp3m_max_layers.argtypes = [C.POINTER(C.c_double), C.c_int,
+                                      C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.c_int,
+                                      C.POINTER(C.c_int32)]
+        m.cp3m_max_layers.restype = C.c_int
+        _model = m
+    return _model
+
+
+def stage_ops(precision: str) -> dict:
+    out = np.zeros(len(STAGES) * len(KEYS))
+    _lib().cp3m_stage_ops(PREC[precision],
+                          out.ctypes.data_as(C.POINTER(C.c_double)))
+    out = out.reshape(len(STAGES), len(KEYS))
+    return {s: dict(zip(KEYS, out[i])) for i, s in enumerate(STAGES)}
+
+
+def max_layers(cosv, earth) -> np.ndarray:
+    c = np.ascontiguousarray(cosv, dtype=np.float64)
+    r = np.ascontiguousarray(earth.radii, dtype=np.float64)
+    d = np.ascontiguousarray(earth.rhos, dtype=np.float64)
+    y = np.ascontiguousarray(earth.yps, dtype=np.float64)
+    M = np.zeros(len(c), dtype=np.int32)
+    dp = C.POINTER(C.c_double)
+    rc = _lib().cp3m_max_layers(c.ctypes.data_as(dp), len(c),
+                                r.ctypes.data_as(dp), d.ctypes.data_as(dp),
+                                y.ctypes.data_as(dp), len(r),
+                                M.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != 0:
+        raise ValueError("bad Earth model")
+    return M
+
+
+def grid_stage_calls(M: np.ndarray, n_e: int, nc: int) -> dict:
+    """Stage calls of one atmospheric job: cosines ordered by decreasing M
+    (the kernel's block order), blocks of nc, every valid energy lane."""
+    Ms = np.sort(M)[::-1]
+    nblocks = (len(Ms) + nc - 1) // nc
+    solves = sum(int(Ms[b * nc]) + 1 for b in range(nblocks))
+    return {
+        "solve": solves * n_e,
+        "middle": int(np.maximum(Ms - 1, 0).sum()) * n_e,
+        "inner": int((Ms >= 1).sum()) * n_e,
+        "finish": len(Ms) * n_e,
+    }
+
+
+def batch_counts(wl, precision: str, nc: int) -> dict:
+    """Total counted work of one launch over an AtmosBatch."""
+    ops = stage_ops(precision)
+    tot = {k: 0.0 for k in KEYS}
+    cache = {}
+    for j in wl.jobs:
+        key = (j.grid, len(j.cosv), len(j.ev))
+        if key not in cache:
+            M = max_layers(j.cosv, wl.earth)
+            cache[key] = grid_stage_calls(M, len(j.ev), nc)
+        calls = cache[key]
+        for st, n in calls.items():
+            for k in KEYS:
+                tot[k] += n * ops[st][k]
+    return tot
+
+
+def beam_counts(n_points: int, precision: str) -> dict:
+    ops = stage_ops(precision)
+    return {k: n_points * ops["beam"][k] for k in KEYS}
+
+
+def measured_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE,
+    WRITE_SIZE), as recorded in profiles/traffic.json, or None."""
+    path = os.path.join(PROFILES, "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data.get(name)

@@ -155,6 +155,76 @@ int cp3_beam_result_device(cp3_beam* p, const void** dptr, int* elem_bytes);
 int cp3_beam_get_stream(cp3_beam* p, void** stream);
 int cp3_beam_last_kernel_ms(cp3_beam* p, float* ms);
 
+
+/* ---------------------------------------------------------------------------
+ * Batched engine: many parameter sets (MCMC walkers), each on a resident grid
+ * of its own size, neutrino and antineutrino, in one persistent launch per
+ * kind (atmospheric, beam), optionally replayed from a captured hipGraph.
+ * No reference counterpart: the reference runs one parameter set per launch
+ * through process-global constant memory (physics.hpp:115-125, 1307-1352).
+ * ------------------------------------------------------------------------- */
+
+typedef struct cp3_batch cp3_batch;
+
+#define CP3_JOB_ATMOS 0
+#define CP3_JOB_BEAM 1
+
+/* One evaluation: a parameter set (setter units) for one neutrino type on a
+ * grid made by cp3_batch_add_*_grid.  kNuType as setMNSMatrix takes it. */
+typedef struct cp3_job {
+  int32_t kind;    /* CP3_JOB_ATMOS or CP3_JOB_BEAM */
+  int32_t grid;    /* grid id of that kind */
+  int32_t nutype;  /* 0 neutrino, 1 antineutrino */
+  int32_t knutype; /* sign of kNuType: delta is negated when < 0 */
+  double theta12, theta13, theta23, dcp;
+  double dm12sq, dm23sq;
+} cp3_job;
+
+int cp3_batch_create(int device, int precision, cp3_batch** out);
+int cp3_batch_destroy(cp3_batch* b);
+/* Resident atmospheric grid: Earth model rows (either order, as setDensity),
+ * cosines, energies and production height, reduced once to the trajectory
+ * table.  Returns its id. */
+int cp3_batch_add_atmos_grid(cp3_batch* b, const double* cosines, int n_cosines,
+                             const double* energies, int n_energies,
+                             const double* radii_km, const double* rhos,
+                             const double* yps, int n_rows, double height_km,
+                             int* grid_id);
+/* Resident beam grid: energies, density (g/cm^3; Ye = 0.5 applied) and path
+ * length (km). */
+int cp3_batch_add_beam_grid(cp3_batch* b, const double* energies,
+                            int n_energies, double rho, double length_km,
+                            int* grid_id);
+/* Sets the job list: allocates the resident outputs, builds the work list
+ * and uploads the per-job constants. */
+int cp3_batch_set_jobs(cp3_batch* b, const cp3_job* jobs, int n_jobs);
+/* New parameter values for the same job list (same kinds and grids): only
+ * the per-job constants are rebuilt and uploaded. */
+int cp3_batch_update_jobs(cp3_batch* b, const cp3_job* jobs, int n_jobs);
+/* Tuning: cosine block (1, 2, 4, 8) and waves per CU of the persistent grid
+ * (-1: one workgroup per work unit); 0 = automatic. */
+int cp3_batch_set_tuning(cp3_batch* b, int cosine_block, int waves_per_cu);
+/* Times every candidate tuning on the current job list (reps launches each)
+ * and keeps the fastest; reports it. */
+int cp3_batch_autotune(cp3_batch* b, int reps, int* cosine_block,
+                       int* waves_per_cu);
+/* Capture the launch into a hipGraph on the next run and replay it after. */
+int cp3_batch_use_graph(cp3_batch* b, int enable);
+/* Enqueue the whole batch on the batch's stream; cp3_batch_wait waits. */
+int cp3_batch_run(cp3_batch* b);
+int cp3_batch_wait(cp3_batch* b);
+/* Milliseconds between events around the last run (kernels only). */
+int cp3_batch_last_ms(cp3_batch* b, float* ms);
+/* Device result of one job ([9][nC][nE] or [9][nE]) and its element size. */
+int cp3_batch_result_device(cp3_batch* b, int job, const void** dptr,
+                            int* elem_bytes);
+/* Copies one job's result to host as double, layout as on the device. */
+int cp3_batch_copy_result(cp3_batch* b, int job, double* host_dst);
+/* Work units, evaluations and output bytes of the current job list. */
+int cp3_batch_stats(cp3_batch* b, int64_t* units, int64_t* evals,
+                    int64_t* out_bytes);
+int cp3_batch_get_stream(cp3_batch* b, void** stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,147 @@
+"""Batched engine: every job's result equals the single-propagator result of
+the same inputs bitwise (a point's arithmetic does not depend on the batch),
+graph replays equal eager runs, parameter updates take effect, and the
+batched FP64 form keeps parity with the oracle."""
+import numpy as np
+import pytest
+
+from conftest import osc_for
+from cudaprob3_amd import workloads as W
+from cudaprob3_amd.batch import ATMOS, BEAM, Batch, make_job
+
+pytestmark = pytest.mark.gpu
+
+
+def single_atmos(job, earth, precision):
+    from cudaprob3_amd import AtmosPropagator
+    t12, t13, t23, d, k, m12, m23 = job.params.for_nutype(job.nutype)
+    with AtmosPropagator(len(job.cosv), len(job.ev), precision=precision) as p:
+        p.setEnergyList(job.ev)
+        p.setCosineList(job.cosv)
+        p.setMNSMatrix(t12, t13, t23, d, k)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(earth.radii, earth.rhos, earth.yps)
+        p.setProductionHeight(22.0)
+        p.calculateProbabilities(job.nutype)
+        return np.transpose(p.probabilities(), (2, 0, 1))  # [9][nC][nE]
+
+
+def single_beam(params, nutype, ev, precision):
+    from cudaprob3_amd import BeamPropagator
+    t12, t13, t23, d, k, m12, m23 = params.for_nutype(nutype)
+    with BeamPropagator(len(ev), precision=precision) as p:
+        p.setEnergyList(ev)
+        p.setMNSMatrix(t12, t13, t23, d, k)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(2.6)
+        p.setPathLength(1300.0)
+        p.calculateProbabilities(nutype)
+        return p.probabilities().T  # [9][nE]
+
+
+def small_mix():
+    """Uneven grids on the 59-shell model, both neutrino types, plus beam."""
+    earth = W.earth_59()
+    sets = W.draw_param_sets(3, 321)
+    grids = {"a": (W.cos_grid(37), W.log_grid(70, 0.5, 50.0)),
+             "b": (W.cos_grid(19), W.log_grid(130, 0.5, 50.0))}
+    jobs = []
+    for i, p in enumerate(sets):
+        g = "a" if i % 2 == 0 else "b"
+        for nt in (0, 1):
+            jobs.append(W.AtmosJob(p, grids[g][0], grids[g][1], nt, grid=g))
+    beam_ev = W.log_grid(300, 0.1, 20.0)
+    return earth, grids, jobs, sets, beam_ev
+
+
+def build(precision, nc=0, waves=0):
+    earth, grids, jobs, sets, beam_ev = small_mix()
+    b = Batch(precision)
+    gid = {k: b.add_atmos_grid(c, e, earth, 22.0) for k, (c, e) in grids.items()}
+    bg = b.add_beam_grid(beam_ev, 2.6, 1300.0)
+    cj = [make_job(ATMOS, gid[j.grid], j.params, j.nutype) for j in jobs]
+    cj += [make_job(BEAM, bg, sets[0], 0), make_job(BEAM, bg, sets[1], 1)]
+    b.set_jobs(cj)
+    if nc or waves:
+        b.set_tuning(nc, waves)
+    return b, earth, jobs, sets, beam_ev
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("tuning", [(1, 0), (4, -1), (8, 2)])
+def test_batch_equals_single(precision, tuning):
+    b, earth, jobs, sets, beam_ev = build(precision, *tuning)
+    b.run()
+    b.wait()
+    for i, job in enumerate(jobs):
+        ref = single_atmos(job, earth, precision)
+        assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
+    n = len(jobs)
+    assert np.array_equal(b.result(n), single_beam(sets[0], 0, beam_ev, precision))
+    assert np.array_equal(b.result(n + 1), single_beam(sets[1], 1, beam_ev, precision))
+    b.close()
+
+
+def test_graph_replays_equal_eager():
+    b, earth, jobs, sets, beam_ev = build("mixed")
+    b.run()
+    b.wait()
+    eager = [b.result(i) for i in range(len(jobs) + 2)]
+    b.use_graph(True)
+    for rep in range(3):
+        b.run()
+        b.wait()
+        for i, ref in enumerate(eager):
+            assert np.array_equal(b.result(i), ref), (rep, i)
+    b.close()
+
+
+def test_update_jobs_and_fp64_parity(oracle):
+    b, earth, jobs, sets, beam_ev = build("fp64")
+    b.use_graph(True)
+    b.run()
+    b.wait()
+    new = W.draw_param_sets(3, 999)
+    # same kinds and grids, new parameter values
+    cj, nj = [], []
+    for i, job in enumerate(jobs):
+        p = new[i // 2]
+        nj.append(W.AtmosJob(p, job.cosv, job.ev, job.nutype, job.grid))
+        cj.append(make_job(ATMOS, 0 if job.grid == "a" else 1, p, job.nutype))
+    cj += [make_job(BEAM, 0, new[0], 0), make_job(BEAM, 0, new[1], 1)]
+    b.update_jobs(cj)
+    b.run()
+    b.wait()
+    for i, job in enumerate(nj):
+        t12, t13, t23, d, k, m12, m23 = job.params.for_nutype(job.nutype)
+        from oracle.loader import Osc
+        ref = oracle.atmos(job.nutype, job.cosv, job.ev, earth.radii,
+                           earth.rhos, earth.yps,
+                           Osc(t12, t13, t23, d, m12, m23, k), 22.0, threads=4)
+        got = np.transpose(b.result(i), (1, 2, 0))
+        assert np.abs(got - ref).max() < 1e-10
+    row = [new[1].th12, new[1].th13, new[1].th23, new[1].dcp, new[1].dm12,
+           new[1].dm23]
+    ref = oracle.beam(1, beam_ev, 2.6, 1300.0, osc_for(row, 1))
+    assert np.abs(b.result(len(nj) + 1).T - ref).max() < 1e-10
+    b.close()
+
+
+def test_autotune_picks_valid():
+    b, *_ = build("mixed")
+    nc, waves = b.autotune(reps=1)
+    assert nc in (1, 2, 4, 8)
+    b.run()
+    b.wait()
+    assert b.last_ms() > 0
+    b.close()
+
+
+def test_batch_errors():
+    from cudaprob3_amd import Cp3Error
+    b = Batch("mixed")
+    with pytest.raises(Cp3Error, match="unknown atmospheric grid"):
+        b.set_jobs([make_job(ATMOS, 3, W.NOMINAL, 0)])
+    with pytest.raises(Cp3Error, match="no jobs"):
+        b.run()
+    b.close()
