@@ -111,6 +111,24 @@ def cpu_baseline(wl, gpu_batch, budget_s=12.0):
     }, worst, float(np.sqrt(sq / max(npts, 1)))
 
 
+def calib_copy(nbytes):
+    """One hipMemcpy device-to-device of nbytes (a plain copy kernel of known
+    traffic: nbytes read, nbytes written)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +138,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a device-to-device copy of this size (for "
+                         "scaling the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--tune", default="auto",
                     help="'auto' (autotune) or 'NC,WAVES' for both forms")
     args = ap.parse_args()
@@ -128,6 +149,9 @@ def main():
               "(the propagator shards by parameter set with no exchange; "
               "see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.calib_copy_mib:
+        calib_copy(args.calib_copy_mib << 20)
 
     wl = W.atmos1024(n_sets=args.sets)
     evals = wl.n_evals

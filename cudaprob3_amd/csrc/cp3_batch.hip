@@ -163,8 +163,10 @@ struct cp3_batch {
   }
 
   void set_jobs(const cp3_job* in, int n) {
+    // validate before touching any state: a rejected list leaves the batch
+    // as it was
+    for (int j = 0; j < n; ++j) check_job(in[j]);
     jobs.assign(in, in + n);
-    for (const cp3_job& j : jobs) check_job(j);
     slot.assign(jobs.size(), 0);
     ajobs.clear();
     bjobs.clear();

@@ -33,6 +33,9 @@
 //      det(Y) conj(y_0 x y_1), with det(Y) = e^{i sum psi} accumulated.
 //   4. cos(t0 -+ 2pi/3) = -cos(t0)/2 +- (sqrt(3)/2) sin(t0): one sincos for
 //      the three roots.
+// Phases: FP64 sincos is an inline Cody-Waite reduction plus Taylor series;
+// the FP32 and mixed forms reduce the phase to quarter revolutions (in FP64
+// for the mixed form) and evaluate FP32 Taylor series of sin and cos.
 #pragma once
 
 #if defined(__HIPCC__)
@@ -152,79 +155,85 @@ CP3_HD void m_sincos(float x, float* s, float* c) {
 }
 #endif
 
-// sin and cos of 2*pi*x for |x| <= 1/2 revolution, FP32.  On the device this
-// is the hardware v_sin_f32 / v_cos_f32 (argument in revolutions).
-template <class T>
-CP3_HD void m_sincos_rev(T x, T* s, T* c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  *s = __builtin_amdgcn_sinf((float)x);
-  *c = __builtin_amdgcn_cosf((float)x);
-#else
-  const double a = 6.283185307179586 * (double)x;
-  *s = (T)sin(a);
-  *c = (T)cos(a);
-#endif
+CP3_HD int m_toint(double x) { return (int)(int64_t)x; }
+CP3_HD int m_toint(float x) { return (int)x; }
+
+// sin and cos of (x + n) * pi/2 for |x| <= 1/2 in FP32: the Taylor series of
+// sin through t^9 and of cos through t^8 on t = x*pi/2 in [-pi/4, pi/4]
+// (truncation below 2e-8), then the quadrant n mod 4.
+CP3_HD void m_sincos_quarter(float x, int n, float* s, float* c) {
+  const float t = x * 1.5707963267948966f;
+  const float z = t * t;
+  float ps = 2.7557319223985893e-06f;
+  ps = ps * z + -0.0001984126984126984f;
+  ps = ps * z + 0.008333333333333333f;
+  ps = ps * z + -0.16666666666666666f;
+  const float sr = (t * z) * ps + t;
+  float pc = 2.48015873015873e-05f;
+  pc = pc * z + -0.001388888888888889f;
+  pc = pc * z + 0.041666666666666664f;
+  pc = pc * z + -0.5f;
+  const float cr = z * pc + 1.0f;
+  const float so = (n & 1) ? cr : sr;
+  const float co = (n & 1) ? sr : cr;
+  *s = (n & 2) ? -so : so;
+  *c = ((n + 1) & 2) ? -co : co;
 }
 
-// e^{i phase} - 1 for a layer, and the phase itself added to `acc`.
-//   FP64 / FP32: phase = om * L in radians.
-//   Mixed: om is in revolutions per km; phase reduced to [-1/2, 1/2] in FP64,
-//   then evaluated in FP32.
+// e^{i phase} - 1 for a layer, with the phase added to the determinant
+// accumulator of the trajectory (acc, quad).
+//   PhaseRad (FP64): om in radians per km; phase = om * L; acc in radians.
+//   PhaseQuarter (FP32, mixed): om in quarter revolutions per km; the phase
+//   q = om * L is reduced in D to k = rint(q), x = q - k, quadrant k mod 4,
+//   and evaluated in FP32; acc keeps the fractions, quad the quadrants.
+template <class D>
+struct PhaseRad {
+  static constexpr double kOmScale = 1.0;
+  CP3_HD static void eval(D om, D L, D& acc, int& quad, D& wr, D& wi) {
+    (void)quad;
+    const D ph = om * L;
+    acc += ph;
+    D s, c;
+    m_sincos(ph, &s, &c);
+    wr = c - D(1);
+    wi = s;
+  }
+  CP3_HD static void unit(D acc, int quad, D& cr, D& ci) {
+    (void)quad;
+    m_sincos(acc, &ci, &cr);
+  }
+};
+
+template <class D, class F>
+struct PhaseQuarter {
+  static constexpr double kOmScale = 0.63661977236758134308;  // 2/pi
+  CP3_HD static void eval(D om, D L, D& acc, int& quad, F& wr, F& wi) {
+    const D q = om * L;
+    const D k = m_rint(q);
+    const D x = q - k;
+    acc += x;
+    const int n = m_toint(k);
+    quad += n;
+    F s, c;
+    m_sincos_quarter(F(x), n, &s, &c);
+    wr = c - F(1);
+    wi = s;
+  }
+  CP3_HD static void unit(D acc, int quad, F& cr, F& ci) {
+    const D k = m_rint(acc);
+    const D x = acc - k;
+    m_sincos_quarter(F(x), quad + m_toint(k), &ci, &cr);
+  }
+};
+
 template <class P>
 struct Phase;
-
 template <>
-struct Phase<PolicyF64> {
-  static constexpr double kOmScale = 1.0;
-  CP3_HD static void eval(double om, double L, double& acc, double& wr,
-                          double& wi) {
-    const double ph = om * L;
-    acc += ph;
-    double s, c;
-    m_sincos(ph, &s, &c);
-    wr = c - 1.0;
-    wi = s;
-  }
-  CP3_HD static void unit(double acc, double& cr, double& ci) {
-    m_sincos(acc, &ci, &cr);
-  }
-};
-
+struct Phase<PolicyF64> : PhaseRad<double> {};
 template <>
-struct Phase<PolicyF32> {
-  static constexpr double kOmScale = 1.0;
-  CP3_HD static void eval(float om, float L, float& acc, float& wr,
-                          float& wi) {
-    const float ph = om * L;
-    acc += ph;
-    float s, c;
-    m_sincos(ph, &s, &c);
-    wr = c - 1.0f;
-    wi = s;
-  }
-  CP3_HD static void unit(float acc, float& cr, float& ci) {
-    m_sincos(acc, &ci, &cr);
-  }
-};
-
+struct Phase<PolicyF32> : PhaseQuarter<float, float> {};
 template <>
-struct Phase<PolicyMixed> {
-  static constexpr double kOmScale = kInvTwoPi;
-  CP3_HD static void eval(double om, double L, double& acc, float& wr,
-                          float& wi) {
-    const double rev = om * L;
-    const double red = rev - m_rint(rev);
-    acc += red;
-    float s, c;
-    m_sincos_rev((float)red, &s, &c);
-    wr = c - 1.0f;
-    wi = s;
-  }
-  CP3_HD static void unit(double acc, float& cr, float& ci) {
-    const double red = acc - m_rint(acc);
-    m_sincos_rev((float)red, &ci, &cr);
-  }
-};
+struct Phase<PolicyMixed> : PhaseQuarter<double, float> {};
 
 // ------------------------------------------------------- per (E, shell)
 
@@ -370,6 +379,7 @@ template <class P>
 struct Traj {
   C3<typename P::F> y0, y1;
   typename P::D acc;
+  int quad;
 };
 
 template <class P>
@@ -392,6 +402,7 @@ CP3_HD void traj_init(Traj<P>& t, bool no_shells) {
     t.y1.im[m] = zero;
   }
   t.acc = typename P::D(0);
+  t.quad = 0;
 }
 
 // A middle shell crossed twice (physics.hpp:971-978): Y -> T Y.
@@ -399,10 +410,10 @@ template <class P>
 CP3_HD void traj_middle(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   using F = typename P::F;
   F wr, wi;
-  Phase<P>::eval(s.oma, L, t.acc, wr, wi);
+  Phase<P>::eval(s.oma, L, t.acc, t.quad, wr, wi);
   rank1(s.va, wr, wi, t.y0);
   rank1(s.va, wr, wi, t.y1);
-  Phase<P>::eval(s.omb, L, t.acc, wr, wi);
+  Phase<P>::eval(s.omb, L, t.acc, t.quad, wr, wi);
   rank1(s.vb, wr, wi, t.y0);
   rank1(s.vb, wr, wi, t.y1);
 }
@@ -431,7 +442,7 @@ CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   using D = typename P::D;
   // third column y2 = det(Y) conj(y0 x y1)
   F er, ei;
-  Phase<P>::unit(t.acc, er, ei);
+  Phase<P>::unit(t.acc, t.quad, er, ei);
   C3<F> y2;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
@@ -456,11 +467,12 @@ CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   dot_cc(t.y1, y2, zr[4], zi[4]);
   dot_cc(y2, y2, zr[5], zi[5]);
   D dummy = D(0);
+  int dq = 0;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const F* v = k == 0 ? s.va : s.vb;
     F wr, wi;
-    Phase<P>::eval(k == 0 ? s.oma : s.omb, L, dummy, wr, wi);
+    Phase<P>::eval(k == 0 ? s.oma : s.omb, L, dummy, dq, wr, wi);
     F ur[3], ui[3], sr[3], si[3];
     dot_cr(t.y0, v, ur[0], ui[0]);
     dot_cr(t.y1, v, ur[1], ui[1]);
@@ -508,10 +520,11 @@ CP3_HD void traj_finish(const Traj<P>& t, const Slab<P>& vac,
       ai[i][j] = zi[i][j];
     }
   D dummy = D(0);
+  int dq = 0;
   for (int k = 0; k < 2; ++k) {
     const F* v = k == 0 ? vac.va : vac.vb;
     F wr, wi;
-    Phase<P>::eval(k == 0 ? vac.oma : vac.omb, Lvac, dummy, wr, wi);
+    Phase<P>::eval(k == 0 ? vac.oma : vac.omb, Lvac, dummy, dq, wr, wi);
     for (int i = 0; i < 3; ++i) {
       const F tr = zr[i][0] * v[0] + zr[i][1] * v[1] + zr[i][2] * v[2];
       const F ti = zi[i][0] * v[0] + zi[i][1] * v[1] + zi[i][2] * v[2];

@@ -120,6 +120,17 @@ inline Cnt<float> m_acos(Cnt<float> x) { g_ops.acos32 += 1; return mk<float>(std
 inline Cnt<double> m_fabs(Cnt<double> x) { return mk<double>(std::fabs(x.v)); }
 inline Cnt<float> m_fabs(Cnt<float> x) { return mk<float>(std::fabs(x.v)); }
 inline Cnt<double> m_rint(Cnt<double> x) { g_ops.rint64 += 1; return mk<double>(std::rint(x.v)); }
+inline Cnt<float> m_rint(Cnt<float> x) { return mk<float>(std::rint(x.v)); }
+template <class T>
+inline int m_toint(Cnt<T> x) { return (int)(int64_t)x.v; }
+// The FP32 quarter-revolution sincos is counted as one sincos call.
+inline void m_sincos_quarter(Cnt<float> x, int n, Cnt<float>* s, Cnt<float>* c) {
+  g_ops.sincos32 += 1;
+  float sv, cv;
+  cp3::m_sincos_quarter(x.v, n, &sv, &cv);
+  s->v = sv;
+  c->v = cv;
+}
 inline void m_sincos(Cnt<double> x, Cnt<double>* s, Cnt<double>* c) {
   g_ops.sincos64 += 1;
   s->v = std::sin(x.v);
@@ -147,53 +158,13 @@ struct CountMixed {
   using F = Cnt<float>;
 };
 
-// The three phase rules of cp3_math.hpp with counted scalars.
+// The phase rules of cp3_math.hpp with counted scalars.
 template <>
-struct Phase<CountF64> {
-  static constexpr double kOmScale = 1.0;
-  static void eval(Cnt<double> om, Cnt<double> L, Cnt<double>& acc, Cnt<double>& wr,
-                   Cnt<double>& wi) {
-    const Cnt<double> ph = om * L;
-    acc += ph;
-    Cnt<double> s, c;
-    m_sincos(ph, &s, &c);
-    wr = c - Cnt<double>(1.0);
-    wi = s;
-  }
-  static void unit(Cnt<double> acc, Cnt<double>& cr, Cnt<double>& ci) { m_sincos(acc, &ci, &cr); }
-};
+struct Phase<CountF64> : PhaseRad<Cnt<double>> {};
 template <>
-struct Phase<CountF32> {
-  static constexpr double kOmScale = 1.0;
-  static void eval(Cnt<float> om, Cnt<float> L, Cnt<float>& acc, Cnt<float>& wr,
-                   Cnt<float>& wi) {
-    const Cnt<float> ph = om * L;
-    acc += ph;
-    Cnt<float> s, c;
-    m_sincos(ph, &s, &c);
-    wr = c - Cnt<float>(1.0);
-    wi = s;
-  }
-  static void unit(Cnt<float> acc, Cnt<float>& cr, Cnt<float>& ci) { m_sincos(acc, &ci, &cr); }
-};
+struct Phase<CountF32> : PhaseQuarter<Cnt<float>, Cnt<float>> {};
 template <>
-struct Phase<CountMixed> {
-  static constexpr double kOmScale = kInvTwoPi;
-  static void eval(Cnt<double> om, Cnt<double> L, Cnt<double>& acc, Cnt<float>& wr,
-                   Cnt<float>& wi) {
-    const Cnt<double> rev = om * L;
-    const Cnt<double> red = rev - m_rint(rev);
-    acc += red;
-    Cnt<float> s, c;
-    m_sincos(Cnt<float>(red), &s, &c);
-    wr = c - Cnt<float>(1.0);
-    wi = s;
-  }
-  static void unit(Cnt<double> acc, Cnt<float>& cr, Cnt<float>& ci) {
-    const Cnt<double> red = acc - m_rint(acc);
-    m_sincos(Cnt<float>(red), &ci, &cr);
-  }
-};
+struct Phase<CountMixed> : PhaseQuarter<Cnt<double>, Cnt<float>> {};
 }  // namespace cp3
 
 namespace {

@@ -142,6 +142,14 @@ def test_batch_errors():
     b = Batch("mixed")
     with pytest.raises(Cp3Error, match="unknown atmospheric grid"):
         b.set_jobs([make_job(ATMOS, 3, W.NOMINAL, 0)])
+    # a rejected job list leaves the batch empty
     with pytest.raises(Cp3Error, match="no jobs"):
         b.run()
+    g = b.add_beam_grid(W.log_grid(10, 0.1, 20.0), 2.6, 1300.0)
+    g2 = b.add_beam_grid(W.log_grid(12, 0.1, 20.0), 2.6, 810.0)
+    b.set_jobs([make_job(BEAM, g, W.NOMINAL, 0)])
+    with pytest.raises(Cp3Error, match="kinds and grids"):
+        b.update_jobs([make_job(BEAM, g2, W.NOMINAL, 0)])
+    with pytest.raises(Cp3Error, match="out of range"):
+        b.result_device(5)
     b.close()

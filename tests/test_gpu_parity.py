@@ -156,3 +156,20 @@ def test_beam_mixed_error(oracle):
             ref = oracle.beam(nt, ev, 2.6, 1300.0, osc_for(row, nt))
             out = run_beam(ev, row, nt, "mixed")
             assert np.abs(out - ref).max() < TOL_MIXED
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_mixed_error_shells59_full_grid(oracle, nutype):
+    """The headline batch's shape: 59 shells, a 400 x 400 grid over
+    0.5-50 GeV, seeded parameter sets; mixed within 1e-5 of the FP64 oracle."""
+    e = W.earth_59()
+    cz, ev = W.cos_grid(400), W.log_grid(400, 0.5, 50.0)
+    for p in W.draw_param_sets(2, 1024):
+        row = [p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+        ref = oracle.atmos(nutype, cz, ev, e.radii, e.rhos, e.yps,
+                           osc_for(row, nutype), 22.0, threads=8)
+        out = run_atmos(cz, ev, e.radii, e.rhos, e.yps, row, nutype, "mixed")
+        d = np.abs(out - ref)
+        print(f"59 shells 400x400 nutype {nutype}: mixed max {d.max():.2e} "
+              f"rms {np.sqrt((d**2).mean()):.2e}")
+        assert d.max() < TOL_MIXED
