@@ -169,7 +169,8 @@ def main():
     t_k = float(np.mean(kms)) / 1e3
     achieved = (counts["f64"] + counts["f32"] * PEAK_F64 / PEAK_F32) / t_k
     out_bytes = evals * 9 * 4
-    traffic = FL.measured_traffic("atmos_batch_mixed")
+    traffic_rec = FL.measured_traffic("atmos_batch_mixed")
+    traffic = traffic_rec["bytes"] if traffic_rec else None
 
     result = {
         "metric": METRIC,
@@ -204,6 +205,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_F64,
             "traffic": traffic,
+            "traffic_detail": traffic_rec,
             "note": "FP64-equivalent rate (F64 + F32*78.6/157.3)/t_kernel of "
                     "the counted FLOPs of the exact kernel decomposition; "
                     "transcendentals counted separately",

@@ -1,0 +1,118 @@
+"""The kernel arithmetic (cp3_math.hpp) run on the CPU through its host build
+(libcp3model.so) against the oracle: the exact rewrites of the Barger method
+keep FP64 within 1e-10 of the reference path, the mixed form within 1e-5; and
+the operation counts used for the roofline are stable."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import golden, osc_for
+from cudaprob3_amd import workloads as W
+from cudaprob3_amd import flops as FL
+
+dp = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def model():
+    from cudaprob3_amd._native import MODEL_SO
+    m = C.CDLL(MODEL_SO)
+    m.cp3m_atmos.argtypes = ([C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, dp,
+                              dp, dp, C.c_int] + [C.c_double] * 4 +
+                             [C.c_int] + [C.c_double] * 3 +
+                             [dp, C.c_char_p, C.c_int])
+    m.cp3m_beam.argtypes = ([C.c_int, C.c_int, dp, C.c_int, C.c_double,
+                             C.c_double] + [C.c_double] * 4 + [C.c_int] +
+                            [C.c_double] * 2 + [dp, C.c_char_p, C.c_int])
+    return m
+
+
+def P(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(dp)
+
+
+def m_atmos(m, prec, nt, cz, ev, earth, row, h=22.0):
+    cz = np.ascontiguousarray(cz, float)
+    ev = np.ascontiguousarray(ev, float)
+    out = np.zeros((len(cz), len(ev), 9))
+    err = C.create_string_buffer(256)
+    t12, t13, t23, d, m12, m23 = [float(x) for x in row]
+    rc = m.cp3m_atmos(prec, nt, P(cz), len(cz), P(ev), len(ev),
+                      P(earth.radii), P(earth.rhos), P(earth.yps),
+                      len(earth.radii), t12, t13, t23, d, -1 if nt else 1,
+                      m12, m23, h, out.ctypes.data_as(dp), err, 256)
+    assert rc == 0, err.value
+    return out
+
+
+NOM = [W.TH12, W.TH13, W.TH23, W.DCP, W.DM12, W.DM23]
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_fp64_rewrite_vs_goldens(model, nutype):
+    g = golden(f"g1_shells12_{'nubar' if nutype else 'nu'}")
+    e = W.EarthModel(g["radii"], g["rhos"], g["yps"])
+    out = m_atmos(model, 0, nutype, g["cosv"], g["ev"], e, NOM)
+    assert np.abs(out - g["prob"]).max() < 1e-10
+
+
+def test_fp64_rewrite_edges(model):
+    g = golden("g4_edges")
+    e = W.EarthModel(g["radii"], g["rhos"], g["yps"])
+    for key in ["nominal", "dm12_zero", "dm23_zero", "dcp_zero", "inverted"]:
+        for nt in (0, 1):
+            out = m_atmos(model, 0, nt, g["cosv"], g["ev"], e,
+                          g[f"params_{key}"])
+            assert np.abs(out - g[f"prob_{key}_{nt}"]).max() < 1e-10, key
+
+
+def test_mixed_rewrite_59_shells(model, oracle):
+    e = W.earth_59()
+    cz, ev = W.cos_grid(96), W.log_grid(96, 0.5, 50.0)
+    for p in W.draw_param_sets(2, 77):
+        row = [p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+        for nt in (0, 1):
+            ref = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps,
+                               osc_for(row, nt), 22.0, threads=4)
+            assert np.abs(m_atmos(model, 2, nt, cz, ev, e, row) - ref).max() < 1e-5
+            assert np.abs(m_atmos(model, 0, nt, cz, ev, e, row) - ref).max() < 1e-10
+
+
+def test_beam_rewrite(model, oracle):
+    g = golden("g2_beam")
+    for nt in (0, 1):
+        for i, par in enumerate(g["params"]):
+            out = np.zeros((len(g["ev"]), 9))
+            err = C.create_string_buffer(256)
+            t12, t13, t23, d, m12, m23 = [float(x) for x in par]
+            rc = model.cp3m_beam(0, nt, P(g["ev"]), len(g["ev"]), 2.6, 1300.0,
+                                 t12, t13, t23, d, -1 if nt else 1, m12, m23,
+                                 out.ctypes.data_as(dp), err, 256)
+            assert rc == 0
+            assert np.abs(out - g["prob"][nt, i]).max() < 1e-10
+
+
+def test_geometry_table_matches_oracle(oracle):
+    e = W.earth_59()
+    cz = W.cos_grid(200)
+    M = FL.max_layers(cz, e)
+    for i in range(0, 200, 17):
+        Mo, _, _ = oracle.segments(cz[i], e.radii, e.rhos, e.yps, 22.0)
+        assert M[i] == Mo
+
+
+def test_stage_counts():
+    f64 = FL.stage_ops("fp64")
+    mixed = FL.stage_ops("mixed")
+    # a middle layer: two rank-one updates of two complex 3-vectors each
+    assert f64["middle"]["f64"] == mixed["middle"]["f32"] + mixed["middle"]["f64"] - 2
+    assert f64["middle"]["sincos64"] == 2 and mixed["middle"]["sincos32"] == 2
+    assert f64["solve"]["acos64"] == 1 and mixed["solve"]["acos64"] == 1
+    assert mixed["solve"]["f32"] == 0  # the eigen-solve stays FP64
+    # the hoisting: fewer solves per trajectory with larger cosine blocks
+    wl = W.atmos1024(n_sets=4)
+    c1 = FL.batch_counts(wl, "mixed", 1)
+    c8 = FL.batch_counts(wl, "mixed", 8)
+    assert c8["acos64"] < c1["acos64"] / 6
+    assert c8["f32"] == c1["f32"]
