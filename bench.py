@@ -142,7 +142,7 @@ def main():
                     help="also run a device-to-device copy of this size (for "
                          "scaling the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--tune", default="auto",
-                    help="'auto' (autotune) or 'NC,WAVES' for both forms")
+                    help="'auto' (autotune) or 'NC,GROUP,WGS_PER_CU' for both forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -159,13 +159,12 @@ def main():
 
     fixed = None
     if args.tune != "auto":
-        nc_s, w_s = args.tune.split(",")
-        fixed = (int(nc_s), int(w_s))
+        fixed = tuple(int(x) for x in args.tune.split(","))
     mixed, tune_m = build_batch("mixed", wl, tuning=fixed, graph=graph)
     ms_step, kms = time_steps(mixed, args.steps, args.warmup)
     value = evals / (ms_step / 1e3)
 
-    counts = FL.batch_counts(wl, "mixed", tune_m[0])
+    counts = FL.batch_counts(wl, "mixed", tune_m[0], tune_m[1])
     t_k = float(np.mean(kms)) / 1e3
     achieved = (counts["f64"] + counts["f32"] * PEAK_F64 / PEAK_F32) / t_k
     out_bytes = evals * 9 * 4
@@ -192,7 +191,8 @@ def main():
                         + ("hipGraph-replayed " if graph else "") + "launch",
             "jobs": len(wl.jobs), "evals_per_step": evals,
             "precision": "mixed",
-            "tuning": {"cosine_block": tune_m[0], "waves_per_cu": tune_m[1]},
+            "tuning": {"cosine_block": tune_m[0], "group_waves": tune_m[1],
+                       "workgroups_per_cu": tune_m[2]},
             "parallelism": "1 GPU",
         },
         "kernel_ms": {"mean": float(np.mean(kms)),
@@ -222,7 +222,7 @@ def main():
         fp64, tune_d = build_batch("fp64", wl, tuning=fixed, graph=graph)
         ms64, kms64 = time_steps(fp64, max(3, args.steps // 2), 1)
         v64 = evals / (ms64 / 1e3)
-        c64 = FL.batch_counts(wl, "fp64", tune_d[0])
+        c64 = FL.batch_counts(wl, "fp64", tune_d[0], tune_d[1])
         t64 = float(np.mean(kms64)) / 1e3
         # mixed vs FP64 on the GPU, over sampled jobs
         worst, sq, n = 0.0, 0.0, 0
@@ -235,7 +235,8 @@ def main():
         result["fp64"] = {
             "value": v64, "ms_per_step": ms64,
             "kernel_ms": float(np.mean(kms64)),
-            "tuning": {"cosine_block": tune_d[0], "waves_per_cu": tune_d[1]},
+            "tuning": {"cosine_block": tune_d[0], "group_waves": tune_d[1],
+                       "workgroups_per_cu": tune_d[2]},
             "frac_of_fp64_peak": c64["f64"] / t64 / PEAK_F64,
         }
         result["mixed_over_fp64"] = value / v64

@@ -62,7 +62,7 @@ _SIGS = {
     "cp3_atmos_result_device": (C.c_int, [_vp, C.POINTER(_vp), _ip]),
     "cp3_atmos_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
     "cp3_atmos_last_kernel_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
-    "cp3_atmos_set_cosine_block": (C.c_int, [_vp, C.c_int]),
+    "cp3_atmos_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
     "cp3_beam_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "cp3_beam_destroy": (C.c_int, [_vp]),
     "cp3_beam_set_energy_list": (C.c_int, [_vp, _dp, C.c_int]),
@@ -98,8 +98,8 @@ _SIGS.update({
     "cp3_batch_add_beam_grid": (C.c_int, [_vp, _dp, C.c_int, C.c_double, C.c_double, _ip]),
     "cp3_batch_set_jobs": (C.c_int, [_vp, C.POINTER(Cp3Job), C.c_int]),
     "cp3_batch_update_jobs": (C.c_int, [_vp, C.POINTER(Cp3Job), C.c_int]),
-    "cp3_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
-    "cp3_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "cp3_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "cp3_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
     "cp3_batch_use_graph": (C.c_int, [_vp, C.c_int]),
     "cp3_batch_run": (C.c_int, [_vp]),
     "cp3_batch_wait": (C.c_int, [_vp]),

@@ -84,15 +84,18 @@ class Batch:
         arr = (Cp3Job * len(jobs))(*jobs)
         N.check(self._lib.cp3_batch_update_jobs(self._h, arr, len(jobs)))
 
-    def set_tuning(self, cosine_block=0, waves_per_cu=0):
+    def set_tuning(self, cosine_block=0, group_waves=0, workgroups_per_cu=0):
+        """(cosine block, waves sharing each solve, resident workgroups per
+        CU or -1 for one workgroup per work unit); 0 = automatic."""
         N.check(self._lib.cp3_batch_set_tuning(self._h, int(cosine_block),
-                                               int(waves_per_cu)))
+                                               int(group_waves),
+                                               int(workgroups_per_cu)))
 
     def autotune(self, reps=2):
-        nc, w = C.c_int(), C.c_int()
+        nc, g, w = C.c_int(), C.c_int(), C.c_int()
         N.check(self._lib.cp3_batch_autotune(self._h, int(reps), C.byref(nc),
-                                             C.byref(w)))
-        return nc.value, w.value
+                                             C.byref(g), C.byref(w)))
+        return nc.value, g.value, w.value
 
     def use_graph(self, enable=True):
         N.check(self._lib.cp3_batch_use_graph(self._h, int(bool(enable))))

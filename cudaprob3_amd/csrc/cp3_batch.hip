@@ -74,8 +74,8 @@ struct cp3_batch {
   int64_t a_units = 0, b_units = 0, evals = 0;
   size_t out_total = 0;
 
-  int tune_nc = 0, tune_waves = 0;  // 0 = automatic
-  int nc = 1, a_grid = 0, b_grid = 0;
+  int tune_nc = 0, tune_w = 0, tune_waves = 0;  // 0 = automatic
+  int nc = 1, w = 1, a_grid = 0, b_grid = 0;
 
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -213,12 +213,13 @@ struct cp3_batch {
   // Unit prefix sums for the chosen cosine block, and grid sizes.
   void choose_tuning() {
     nc = tune_nc ? tune_nc : auto_nc();
+    w = tune_w ? tune_w : 4;
     int64_t u = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
       if (jobs[j].kind != CP3_JOB_ATMOS) continue;
       cp3::AtmosJob& a = ajobs[(size_t)slot[j]];
       a.unit_begin = u;
-      u += cp3::atmos_units(agrids[(size_t)jobs[j].grid]->dev(), nc);
+      u += cp3::atmos_units(agrids[(size_t)jobs[j].grid]->dev(), nc, w);
     }
     a_units = u;
     u = 0;
@@ -240,8 +241,8 @@ struct cp3_batch {
       int64_t u = 0;
       for (size_t j = 0; j < jobs.size(); ++j)
         if (jobs[j].kind == CP3_JOB_ATMOS)
-          u += cp3::atmos_units(agrids[(size_t)jobs[j].grid]->dev(), c);
-      if (u >= (int64_t)cus * 4 * 8) return c;
+          u += cp3::atmos_units(agrids[(size_t)jobs[j].grid]->dev(), c, 4);
+      if (u * 4 >= (int64_t)cus * 4 * 8) return c;
     }
     return 1;
   }
@@ -251,7 +252,7 @@ struct cp3_batch {
     if (waves < 0 || units <= (int64_t)cus) return (int)std::min<int64_t>(units, 0x7fffffff);
     int occ = 0;
     if (atmos)
-      hip_check(cp3::atmos_batch_occupancy(precision, nc, &occ), "occupancy");
+      hip_check(cp3::atmos_batch_occupancy(precision, nc, w, &occ), "occupancy");
     else
       hip_check(cp3::beam_batch_occupancy(precision, &occ), "occupancy");
     if (occ <= 0) occ = 1;
@@ -261,7 +262,7 @@ struct cp3_batch {
 
   void enqueue() {
     if (a_units > 0)
-      hip_check(cp3::launch_atmos_batch(precision, nc, (const cp3::AtmosJob*)d_ajobs.p,
+      hip_check(cp3::launch_atmos_batch(precision, nc, w, (const cp3::AtmosJob*)d_ajobs.p,
                                         (int)ajobs.size(), a_units, a_grid, stream),
                 "batched atmospheric launch");
     if (b_units > 0)
@@ -312,43 +313,48 @@ struct cp3_batch {
     return ms;
   }
 
-  void autotune(int reps, int* best_nc, int* best_waves) {
+  void autotune(int reps, int* best_nc, int* best_w, int* best_waves) {
     if (jobs.empty()) throw Error(CP3_ERR_STATE, "no jobs set");
     if (reps < 1) reps = 1;
     const bool had_graph = use_graph;
     use_graph = false;
     drop_graph();
     float best = 1e30f;
-    int bn = 1, bw = 0;
-    const int ncs[4] = {1, 2, 4, 8};
-    const int waves[5] = {0, -1, 2, 4, 8};
+    int bn = 1, bw = 1, bwv = 0;
     const bool any_atmos = !ajobs.empty();
-    for (int ci = 0; ci < (any_atmos ? 4 : 1); ++ci) {
-      for (int wi = 0; wi < 5; ++wi) {
-        tune_nc = any_atmos ? ncs[ci] : 0;
-        tune_waves = waves[wi];
-        choose_tuning();
-        build_tables(false);
-        run();  // warm-up
-        wait();
-        float tot = 0.f;
-        for (int r = 0; r < reps; ++r) {
-          run();
-          tot += last_ms();
+    const int ncs[4] = {1, 2, 4, 8};
+    const int gws[3] = {1, 2, 4};
+    const int waves[2] = {0, -1};
+    for (int ci = 0; ci < (any_atmos ? 4 : 1); ++ci)
+      for (int gi = 0; gi < (any_atmos ? 3 : 1); ++gi)
+        for (int wi = 0; wi < 2; ++wi) {
+          tune_nc = any_atmos ? ncs[ci] : 0;
+          tune_w = any_atmos ? gws[gi] : 0;
+          tune_waves = waves[wi];
+          choose_tuning();
+          build_tables(false);
+          run();  // warm-up
+          wait();
+          float tot = 0.f;
+          for (int r = 0; r < reps; ++r) {
+            run();
+            tot += last_ms();
+          }
+          if (tot / reps < best) {
+            best = tot / reps;
+            bn = tune_nc;
+            bw = tune_w;
+            bwv = tune_waves;
+          }
         }
-        if (tot / reps < best) {
-          best = tot / reps;
-          bn = tune_nc;
-          bw = tune_waves;
-        }
-      }
-    }
     tune_nc = bn;
-    tune_waves = bw;
+    tune_w = bw;
+    tune_waves = bwv;
     choose_tuning();
     build_tables(false);
     use_graph = had_graph;
     if (best_nc) *best_nc = nc;
+    if (best_w) *best_w = w;
     if (best_waves) *best_waves = tune_waves;
   }
 };
@@ -467,12 +473,15 @@ int cp3_batch_update_jobs(cp3_batch* b, const cp3_job* jobs, int n) {
   });
 }
 
-int cp3_batch_set_tuning(cp3_batch* b, int nc, int waves) {
+int cp3_batch_set_tuning(cp3_batch* b, int nc, int w, int waves) {
   return guard([&] {
     need(b);
     if (nc != 0 && !cp3::valid_nc(nc))
       throw Error(CP3_ERR_INVALID, "cosine block must be 0, 1, 2, 4 or 8");
+    if (w != 0 && !cp3::valid_group(w))
+      throw Error(CP3_ERR_INVALID, "group waves must be 0, 1, 2 or 4");
     b->tune_nc = nc;
+    b->tune_w = w;
     b->tune_waves = waves;
     if (!b->jobs.empty()) {
       b->choose_tuning();
@@ -482,8 +491,8 @@ int cp3_batch_set_tuning(cp3_batch* b, int nc, int waves) {
   });
 }
 
-int cp3_batch_autotune(cp3_batch* b, int reps, int* nc, int* waves) {
-  return guard([&] { need(b)->autotune(reps, nc, waves); });
+int cp3_batch_autotune(cp3_batch* b, int reps, int* nc, int* w, int* waves) {
+  return guard([&] { need(b)->autotune(reps, nc, w, waves); });
 }
 
 int cp3_batch_use_graph(cp3_batch* b, int enable) {

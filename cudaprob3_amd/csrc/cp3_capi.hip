@@ -63,18 +63,24 @@ struct OscArgs {
 };
 
 // Largest cosine block per precision that keeps the state in registers at a
-// useful occupancy; the work-unit count decides below that.
-int auto_nc(int precision, const cp3::GridDev& g) {
+// useful occupancy, while leaving at least 4096 waves; a single grid then
+// shares each eigen-solve between two waves.
+void auto_tuning(int precision, const cp3::GridDev& g, int* nc, int* w) {
   const int cap = precision == CP3_FP64 ? 2 : (precision == CP3_FP32 ? 8 : 4);
-  for (int nc = cap; nc > 1; nc /= 2)
-    if (cp3::atmos_units(g, nc) >= 4096) return nc;
-  return 1;
+  *w = 2;
+  for (int c = cap; c > 1; c /= 2)
+    if (cp3::atmos_units(g, c, *w) * *w >= 4096) {
+      *nc = c;
+      return;
+    }
+  *nc = 1;
+  if (cp3::atmos_units(g, 1, *w) * *w < 4096) *w = 1;
 }
 
 }  // namespace
 
 struct cp3_atmos {
-  int dev = 0, nC = 0, nE = 0, precision = CP3_FP64, nc_force = 0;
+  int dev = 0, nC = 0, nE = 0, precision = CP3_FP64, nc_force = 0, w_force = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool launched = false;
@@ -143,10 +149,14 @@ struct cp3_atmos {
     job.g = grid();
     job.out = d_out.p;
     job.unit_begin = 0;
-    const int nc = nc_force ? nc_force : auto_nc(precision, job.g);
+    int nc = 1, w = 1;
+    auto_tuning(precision, job.g, &nc, &w);
+    if (nc_force) nc = nc_force;
+    if (w_force) w = w_force;
     results_on_host = false;
     hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
-    hip_check(cp3::launch_atmos_single(precision, nc, job, stream), "atmospheric kernel launch");
+    hip_check(cp3::launch_atmos_single(precision, nc, w, job, stream),
+              "atmospheric kernel launch");
     hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
     launched = true;
   }
@@ -483,12 +493,15 @@ int cp3_atmos_last_kernel_ms(cp3_atmos* p, float* ms) {
   });
 }
 
-int cp3_atmos_set_cosine_block(cp3_atmos* p, int nc) {
+int cp3_atmos_set_tuning(cp3_atmos* p, int nc, int w) {
   return guard([&] {
     need(p);
     if (nc != 0 && !cp3::valid_nc(nc))
       throw Error(CP3_ERR_INVALID, "cosine block must be 0, 1, 2, 4 or 8");
+    if (w != 0 && !cp3::valid_group(w))
+      throw Error(CP3_ERR_INVALID, "group waves must be 0, 1, 2 or 4");
     p->nc_force = nc;
+    p->w_force = w;
   });
 }
 

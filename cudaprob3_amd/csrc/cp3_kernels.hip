@@ -1,13 +1,17 @@
 // HIP kernels for gfx950: atmospheric and beam propagation.
 //
-// Atmospheric work unit = one wave of 64 lanes: 64 consecutive energies of one
-// job times a block of NC cosines.  Each lane owns one energy and carries NC
-// trajectories.  The shell loop is the outer loop: each (energy, shell)
-// eigen-solution is computed once and applied to every trajectory of the
-// block that crosses the shell.  Everything that depends only on the cosine
-// or the shell (segment lengths, densities, shell counts) is wave-uniform and
-// read with scalar loads; the 9 probabilities are stored as 9 coalesced rows
-// of the [9][nC][nE] output.
+// Atmospheric work unit = one workgroup of W waves: 64 consecutive energies of
+// one job times W blocks of NC cosines (cosines ordered by decreasing shell
+// count, so a unit's trajectories cross similar depths).  Each lane owns one
+// energy and carries NC trajectories.  The shell loop is the outer loop: each
+// (energy, shell) eigen-solution is computed once per unit and applied to
+// every trajectory that crosses the shell.  With W > 1 the waves take turns
+// solving the shells of a chunk of W shells and publish the solutions in LDS
+// (double-buffered, one barrier per chunk), so a solve serves W * NC
+// trajectories.  Everything that depends only on the cosine or the shell
+// (segment lengths, densities, shell counts) is wave-uniform and read with
+// scalar loads; the 9 probabilities are stored as 9 coalesced rows of the
+// [9][nC][nE] output.
 #include <hip/hip_runtime.h>
 
 #include "cp3_launch.hpp"
@@ -24,40 +28,20 @@ struct OutT<PolicyF64> {
   using type = double;
 };
 
+// Applies shell i's slab to the NC trajectories of a wave.  Mmin is the
+// smallest shell count of the wave's trajectories: shells below Mmin - 1 are
+// middle shells of all of them and run branch-free, so the NC independent
+// update chains interleave.
 template <class P, int NC>
-__device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local) {
+__device__ __forceinline__ void apply_shell(Traj<P> (&t)[NC], const Slab<P>& s,
+                                            const GridDev& g, const int (&cidx)[NC],
+                                            const int (&Mc)[NC], int Mmin, int i) {
   using D = typename P::D;
-  using F = typename P::F;
-  using O = typename OutT<P>::type;
-  const GridDev& g = job.g;
-  const int ntile = (g.nE + kWave - 1) / kWave;
-  const int cb = (int)(local / ntile);
-  const int et = (int)(local - (int64_t)cb * ntile);
-  const int e = et * kWave + (int)threadIdx.x;
-  const bool e_ok = e < g.nE;
-  const D E = (D)g.ev[e_ok ? e : g.nE - 1];
-  const D inv = D(1) / E;
-
-  int cidx[NC];
-  int Mc[NC];
-  bool c_ok[NC];
-  int Mmax = 0;
+  if (i < Mmin - 1) {
 #pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    const int k = cb * NC + j;
-    c_ok[j] = k < g.nC;
-    cidx[j] = g.perm[c_ok[j] ? k : g.nC - 1];
-    Mc[j] = g.M[cidx[j]];
-    Mmax = Mc[j] > Mmax ? Mc[j] : Mmax;
-  }
-
-  Traj<P> t[NC];
-#pragma unroll
-  for (int j = 0; j < NC; ++j) traj_init<P>(t[j], Mc[j] == 0);
-
-  for (int i = 0; i < Mmax; ++i) {
-    Slab<P> s;
-    solve_slab<P>(job.sc, E, inv, (D)g.rye[i], s);
+    for (int j = 0; j < NC; ++j)
+      traj_middle<P>(t[j], s, (D)g.seg[(int64_t)cidx[j] * g.S + i]);
+  } else {
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
       if (i < Mc[j] - 1) {
@@ -66,6 +50,113 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local) {
         traj_inner<P>(t[j], s, (D)g.seg[(int64_t)cidx[j] * g.S + i]);
       }
     }
+  }
+}
+
+// LDS staging of one chunk of W slabs, two buffers; struct of arrays with one
+// entry per lane so every access is a conflict-free contiguous row.
+template <class P, int W>
+struct SlabStage {
+  typename P::F v[2][W][6][kWave];
+  typename P::D om[2][W][2][kWave];
+};
+
+template <class P, int W>
+__device__ __forceinline__ void stage_put(SlabStage<P, W>& st, int buf, int w,
+                                          int lane, const Slab<P>& s) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    st.v[buf][w][k][lane] = s.va[k];
+    st.v[buf][w][3 + k][lane] = s.vb[k];
+  }
+  st.om[buf][w][0][lane] = s.oma;
+  st.om[buf][w][1][lane] = s.omb;
+}
+
+template <class P, int W>
+__device__ __forceinline__ void stage_get(const SlabStage<P, W>& st, int buf,
+                                          int w, int lane, Slab<P>& s) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    s.va[k] = st.v[buf][w][k][lane];
+    s.vb[k] = st.v[buf][w][3 + k][lane];
+  }
+  s.oma = st.om[buf][w][0][lane];
+  s.omb = st.om[buf][w][1][lane];
+}
+
+// Dummy staging type for W == 1 (no LDS).
+struct NoStage {};
+
+template <class P, int NC, int W, class Stage>
+__device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
+                                           Stage& st) {
+  using D = typename P::D;
+  using F = typename P::F;
+  using O = typename OutT<P>::type;
+  const GridDev& g = job.g;
+  const int lane = (int)threadIdx.x % kWave;
+  const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+  const int ntile = (g.nE + kWave - 1) / kWave;
+  const int sb = (int)(local / ntile);
+  const int et = (int)(local - (int64_t)sb * ntile);
+  const int e = et * kWave + lane;
+  const bool e_ok = e < g.nE;
+  const D E = (D)g.ev[e_ok ? e : g.nE - 1];
+  const D inv = D(1) / E;
+
+  const int base = sb * (W * NC);
+  int cidx[NC];
+  int Mc[NC];
+  bool c_ok[NC];
+  int Mmax = 0, Mmin = 1 << 30;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const int k = base + wave * NC + j;
+    c_ok[j] = k < g.nC;
+    cidx[j] = g.perm[c_ok[j] ? k : g.nC - 1];
+    Mc[j] = c_ok[j] ? g.M[cidx[j]] : 0;
+    Mmax = Mc[j] > Mmax ? Mc[j] : Mmax;
+    Mmin = Mc[j] < Mmin ? Mc[j] : Mmin;
+  }
+
+  Traj<P> t[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) traj_init<P>(t[j], Mc[j] == 0);
+
+  if constexpr (W == 1) {
+    (void)st;
+    for (int i = 0; i < Mmax; ++i) {
+      Slab<P> s;
+      solve_slab<P>(job.sc, E, inv, (D)g.rye[i], s);
+      apply_shell<P, NC>(t, s, g, cidx, Mc, Mmin, i);
+    }
+  } else {
+    // deepest trajectory of the unit: its first cosine (decreasing order)
+    const int Mwg = g.M[g.perm[base]];
+    int buf = 0;
+    for (int i0 = 0; i0 < Mwg; i0 += W) {
+      const int is = i0 + wave;
+      if (is < Mwg) {
+        Slab<P> s;
+        solve_slab<P>(job.sc, E, inv, (D)g.rye[is], s);
+        stage_put<P, W>(st, buf, wave, lane, s);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int i = i0 + w;
+        if (i < Mmax) {
+          Slab<P> s;
+          stage_get<P, W>(st, buf, w, lane, s);
+          apply_shell<P, NC>(t, s, g, cidx, Mc, Mmin, i);
+        }
+      }
+      buf ^= 1;
+    }
+    // the next unit of a persistent workgroup restarts at buffer 0: every
+    // wave must be done reading this unit's last chunk first
+    __syncthreads();
   }
 
   Slab<P> vac;
@@ -84,9 +175,19 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local) {
   }
 }
 
-template <class P, int NC>
-__global__ void __launch_bounds__(kWave) atmos_single_kernel(AtmosJob job) {
-  atmos_unit<P, NC>(job, (int64_t)blockIdx.x);
+template <class P, int W>
+struct StageOf {
+  using type = SlabStage<P, W>;
+};
+template <class P>
+struct StageOf<P, 1> {
+  using type = NoStage;
+};
+
+template <class P, int NC, int W>
+__global__ void __launch_bounds__(kWave * W) atmos_single_kernel(AtmosJob job) {
+  __shared__ typename StageOf<P, W>::type st;
+  atmos_unit<P, NC, W>(job, (int64_t)blockIdx.x, st);
 }
 
 // Finds the job owning work unit u: jobs[j].unit_begin <= u < jobs[j+1]...
@@ -103,15 +204,16 @@ __device__ __forceinline__ int find_job_atmos(const AtmosJob* jobs, int njobs,
   return lo;
 }
 
-// Persistent form: a fixed grid of waves walks the flattened work list of
-// every job of the batch with a static stride.
-template <class P, int NC>
-__global__ void __launch_bounds__(kWave) atmos_batch_kernel(
+// Batched form over the flattened work list of every job: one workgroup per
+// work unit, or a persistent grid walking the list with a static stride.
+template <class P, int NC, int W>
+__global__ void __launch_bounds__(kWave * W) atmos_batch_kernel(
     const AtmosJob* __restrict__ jobs, int njobs, int64_t nunits) {
+  __shared__ typename StageOf<P, W>::type st;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const int j = find_job_atmos(jobs, njobs, u);
     const AtmosJob& job = jobs[j];
-    atmos_unit<P, NC>(job, u - job.unit_begin);
+    atmos_unit<P, NC, W>(job, u - job.unit_begin, st);
   }
 }
 
@@ -156,56 +258,61 @@ __global__ void __launch_bounds__(kWave) beam_batch_kernel(
 
 // ------------------------------------------------------------- launchers
 
+#define CP3_NCW_CASES(X)                                                   \
+  X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(1, 2) X(2, 2) X(4, 2) X(8, 2) X(1, 4) \
+  X(2, 4) X(4, 4) X(8, 4)
+
 template <class P>
-static hipError_t atmos_single_nc(int nc, const AtmosJob& job,
-                                  hipStream_t st) {
-  const int64_t n = atmos_units(job.g, nc);
+static hipError_t atmos_single_p(int nc, int w, const AtmosJob& job,
+                                 hipStream_t st) {
+  const int64_t n = atmos_units(job.g, nc, w);
   if (n <= 0) return hipSuccess;
   if (n > 0x7fffffff) return hipErrorInvalidConfiguration;
-  const dim3 grid((unsigned)n), block(kWave);
-  switch (nc) {
-    case 1: atmos_single_kernel<P, 1><<<grid, block, 0, st>>>(job); break;
-    case 2: atmos_single_kernel<P, 2><<<grid, block, 0, st>>>(job); break;
-    case 4: atmos_single_kernel<P, 4><<<grid, block, 0, st>>>(job); break;
-    case 8: atmos_single_kernel<P, 8><<<grid, block, 0, st>>>(job); break;
-    default: return hipErrorInvalidValue;
+  const dim3 grid((unsigned)n);
+#define X(NC_, W_)                                                        \
+  if (nc == NC_ && w == W_) {                                             \
+    atmos_single_kernel<P, NC_, W_><<<grid, dim3(kWave * W_), 0, st>>>(job); \
+    return hipGetLastError();                                             \
   }
-  return hipGetLastError();
+  CP3_NCW_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
 }
 
-hipError_t launch_atmos_single(int precision, int nc, const AtmosJob& job,
+hipError_t launch_atmos_single(int precision, int nc, int w, const AtmosJob& job,
                                hipStream_t st) {
   switch (precision) {
-    case kFP64: return atmos_single_nc<PolicyF64>(nc, job, st);
-    case kFP32: return atmos_single_nc<PolicyF32>(nc, job, st);
-    case kMixed: return atmos_single_nc<PolicyMixed>(nc, job, st);
+    case kFP64: return atmos_single_p<PolicyF64>(nc, w, job, st);
+    case kFP32: return atmos_single_p<PolicyF32>(nc, w, job, st);
+    case kMixed: return atmos_single_p<PolicyMixed>(nc, w, job, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <class P>
-static hipError_t atmos_batch_nc(int nc, const AtmosJob* jobs, int njobs,
-                                 int64_t nunits, int grid, hipStream_t st) {
+static hipError_t atmos_batch_p(int nc, int w, const AtmosJob* jobs, int njobs,
+                                int64_t nunits, int grid, hipStream_t st) {
   if (nunits <= 0 || njobs <= 0) return hipSuccess;
-  const dim3 g((unsigned)grid), b(kWave);
-  switch (nc) {
-    case 1: atmos_batch_kernel<P, 1><<<g, b, 0, st>>>(jobs, njobs, nunits); break;
-    case 2: atmos_batch_kernel<P, 2><<<g, b, 0, st>>>(jobs, njobs, nunits); break;
-    case 4: atmos_batch_kernel<P, 4><<<g, b, 0, st>>>(jobs, njobs, nunits); break;
-    case 8: atmos_batch_kernel<P, 8><<<g, b, 0, st>>>(jobs, njobs, nunits); break;
-    default: return hipErrorInvalidValue;
+  const dim3 g((unsigned)grid);
+#define X(NC_, W_)                                                             \
+  if (nc == NC_ && w == W_) {                                                  \
+    atmos_batch_kernel<P, NC_, W_><<<g, dim3(kWave * W_), 0, st>>>(jobs, njobs, \
+                                                                   nunits);    \
+    return hipGetLastError();                                                  \
   }
-  return hipGetLastError();
+  CP3_NCW_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
 }
 
-hipError_t launch_atmos_batch(int precision, int nc, const AtmosJob* jobs,
+hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* jobs,
                               int njobs, int64_t nunits, int grid,
                               hipStream_t st) {
   if (grid <= 0) return hipErrorInvalidConfiguration;
   switch (precision) {
-    case kFP64: return atmos_batch_nc<PolicyF64>(nc, jobs, njobs, nunits, grid, st);
-    case kFP32: return atmos_batch_nc<PolicyF32>(nc, jobs, njobs, nunits, grid, st);
-    case kMixed: return atmos_batch_nc<PolicyMixed>(nc, jobs, njobs, nunits, grid, st);
+    case kFP64: return atmos_batch_p<PolicyF64>(nc, w, jobs, njobs, nunits, grid, st);
+    case kFP32: return atmos_batch_p<PolicyF32>(nc, w, jobs, njobs, nunits, grid, st);
+    case kMixed: return atmos_batch_p<PolicyMixed>(nc, w, jobs, njobs, nunits, grid, st);
     default: return hipErrorInvalidValue;
   }
 }
@@ -239,21 +346,21 @@ hipError_t launch_beam_batch(int precision, const BeamJob* jobs, int njobs,
 }
 
 template <class P>
-static hipError_t atmos_occ(int nc, int* n) {
-  switch (nc) {
-    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 1>, kWave, 0);
-    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 2>, kWave, 0);
-    case 4: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 4>, kWave, 0);
-    case 8: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_batch_kernel<P, 8>, kWave, 0);
-    default: return hipErrorInvalidValue;
-  }
+static hipError_t atmos_occ(int nc, int w, int* n) {
+#define X(NC_, W_)                                                     \
+  if (nc == NC_ && w == W_)                                            \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(               \
+        n, atmos_batch_kernel<P, NC_, W_>, kWave * W_, 0);
+  CP3_NCW_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
 }
 
-hipError_t atmos_batch_occupancy(int precision, int nc, int* n) {
+hipError_t atmos_batch_occupancy(int precision, int nc, int w, int* n) {
   switch (precision) {
-    case kFP64: return atmos_occ<PolicyF64>(nc, n);
-    case kFP32: return atmos_occ<PolicyF32>(nc, n);
-    case kMixed: return atmos_occ<PolicyMixed>(nc, n);
+    case kFP64: return atmos_occ<PolicyF64>(nc, w, n);
+    case kFP32: return atmos_occ<PolicyF32>(nc, w, n);
+    case kMixed: return atmos_occ<PolicyMixed>(nc, w, n);
     default: return hipErrorInvalidValue;
   }
 }

@@ -49,20 +49,21 @@ struct BeamJob {
 
 constexpr int kWave = 64;
 
-// Work units of one job: (cosine block of nc) x (64-energy tile).
-inline int64_t atmos_units(const GridDev& g, int nc) {
-  const int64_t cb = (g.nC + nc - 1) / nc;
+// Work units of one job: (w cosine blocks of nc) x (64-energy tile).
+inline int64_t atmos_units(const GridDev& g, int nc, int w = 1) {
+  const int64_t cb = (g.nC + nc * w - 1) / (nc * w);
   const int64_t et = (g.nE + kWave - 1) / kWave;
   return cb * et;
 }
 inline int64_t beam_units(int nE) { return (nE + kWave - 1) / kWave; }
 
-// Supported cosine-block sizes for the atmospheric kernel.
+// Supported cosine-block sizes and waves per workgroup.
 inline bool valid_nc(int nc) { return nc == 1 || nc == 2 || nc == 4 || nc == 8; }
+inline bool valid_group(int w) { return w == 1 || w == 2 || w == 4; }
 
-hipError_t launch_atmos_single(int precision, int nc, const AtmosJob& job,
+hipError_t launch_atmos_single(int precision, int nc, int w, const AtmosJob& job,
                                hipStream_t stream);
-hipError_t launch_atmos_batch(int precision, int nc, const AtmosJob* d_jobs,
+hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* d_jobs,
                               int njobs, int64_t nunits, int grid,
                               hipStream_t stream);
 hipError_t launch_beam_single(int precision, const BeamJob& job,
@@ -71,7 +72,7 @@ hipError_t launch_beam_batch(int precision, const BeamJob* d_jobs, int njobs,
                              int64_t nunits, int grid, hipStream_t stream);
 
 // Resident 64-lane workgroups per CU for the batched kernels (occupancy).
-hipError_t atmos_batch_occupancy(int precision, int nc, int* blocks_per_cu);
+hipError_t atmos_batch_occupancy(int precision, int nc, int w, int* blocks_per_cu);
 hipError_t beam_batch_occupancy(int precision, int* blocks_per_cu);
 
 }  // namespace cp3

@@ -4,10 +4,11 @@ calls of the exact decomposition the kernels execute.
 Per-stage counts come from the host build of the kernel arithmetic run with an
 operation-counting scalar type (``cp3m_stage_ops`` in libcp3model.so):
 add/sub/mul/div/sqrt count 1, a fused multiply-add 2; sincos/acos are counted
-apart.  A launch's total follows the kernel's work list: per cosine block of
-``nc`` trajectories and per valid energy lane, one eigen-solve per shell up to
-the block's deepest shell plus one for the atmosphere; per trajectory, M-1
-middle shells, one innermost shell (M >= 1) and the finish.
+apart.  A launch's total follows the kernel's work list: per work unit of
+``w`` waves x ``nc`` trajectories and per valid energy lane, one eigen-solve
+per shell down to the unit's deepest trajectory plus one atmosphere solve per
+wave; per trajectory, M-1 middle shells, one innermost shell (M >= 1) and the
+finish.
 """
 from __future__ import annotations
 
@@ -71,12 +72,15 @@ def max_layers(cosv, earth) -> np.ndarray:
     return M
 
 
-def grid_stage_calls(M: np.ndarray, n_e: int, nc: int) -> dict:
+def grid_stage_calls(M: np.ndarray, n_e: int, nc: int, w: int = 1) -> dict:
     """Stage calls of one atmospheric job: cosines ordered by decreasing M
-    (the kernel's block order), blocks of nc, every valid energy lane."""
+    (the kernel's block order), work units of w waves x nc trajectories,
+    every valid energy lane: one solve per shell down to the unit's deepest
+    trajectory plus one atmosphere solve per wave."""
     Ms = np.sort(M)[::-1]
-    nblocks = (len(Ms) + nc - 1) // nc
-    solves = sum(int(Ms[b * nc]) + 1 for b in range(nblocks))
+    per = nc * w
+    nunits = (len(Ms) + per - 1) // per
+    solves = sum(int(Ms[u * per]) + w for u in range(nunits))
     return {
         "solve": solves * n_e,
         "middle": int(np.maximum(Ms - 1, 0).sum()) * n_e,
@@ -85,7 +89,7 @@ def grid_stage_calls(M: np.ndarray, n_e: int, nc: int) -> dict:
     }
 
 
-def batch_counts(wl, precision: str, nc: int) -> dict:
+def batch_counts(wl, precision: str, nc: int, w: int = 1) -> dict:
     """Total counted work of one launch over an AtmosBatch."""
     ops = stage_ops(precision)
     tot = {k: 0.0 for k in KEYS}
@@ -94,7 +98,7 @@ def batch_counts(wl, precision: str, nc: int) -> dict:
         key = (j.grid, len(j.cosv), len(j.ev))
         if key not in cache:
             M = max_layers(j.cosv, wl.earth)
-            cache[key] = grid_stage_calls(M, len(j.ev), nc)
+            cache[key] = grid_stage_calls(M, len(j.ev), nc, w)
         calls = cache[key]
         for st, n in calls.items():
             for k in KEYS:

@@ -108,8 +108,11 @@ class AtmosPropagator:
         N.check(self._lib.cp3_atmos_get_nlayer_boundaries(self._h, C.byref(n)))
         return n.value
 
-    def setCosineBlock(self, nc: int):
-        N.check(self._lib.cp3_atmos_set_cosine_block(self._h, int(nc)))
+    def setTuning(self, cosine_block: int = 0, group_waves: int = 0):
+        """Trajectories per lane and waves sharing each eigen-solve (0 =
+        automatic); results do not depend on them."""
+        N.check(self._lib.cp3_atmos_set_tuning(self._h, int(cosine_block),
+                                               int(group_waves)))
 
     # calculation and results
     def calculateProbabilities(self, nutype=Neutrino):

@@ -123,8 +123,10 @@ int cp3_atmos_result_device(cp3_atmos* p, const void** dptr, int* elem_bytes);
 int cp3_atmos_get_stream(cp3_atmos* p, void** stream);
 /* Milliseconds of the last kernel, measured with HIP events around it. */
 int cp3_atmos_last_kernel_ms(cp3_atmos* p, float* ms);
-/* Cosine-block size of the kernel (1, 2, 4 or 8; 0 = automatic). */
-int cp3_atmos_set_cosine_block(cp3_atmos* p, int nc);
+/* Kernel tuning: trajectories per lane (cosine block 1, 2, 4 or 8) and waves
+ * per workgroup sharing each eigen-solve through LDS (1, 2 or 4); 0 =
+ * automatic.  Results do not depend on it (bitwise). */
+int cp3_atmos_set_tuning(cp3_atmos* p, int cosine_block, int group_waves);
 
 /* ---------------------------------------------------------------------------
  * Beam propagator (one energy list, one parameter set per calculate).
@@ -201,13 +203,16 @@ int cp3_batch_set_jobs(cp3_batch* b, const cp3_job* jobs, int n_jobs);
 /* New parameter values for the same job list (same kinds and grids): only
  * the per-job constants are rebuilt and uploaded. */
 int cp3_batch_update_jobs(cp3_batch* b, const cp3_job* jobs, int n_jobs);
-/* Tuning: cosine block (1, 2, 4, 8) and waves per CU of the persistent grid
- * (-1: one workgroup per work unit); 0 = automatic. */
-int cp3_batch_set_tuning(cp3_batch* b, int cosine_block, int waves_per_cu);
+/* Tuning: cosine block (1, 2, 4, 8), waves per workgroup sharing each
+ * eigen-solve (1, 2, 4) and resident workgroups per CU of the persistent grid
+ * (-1: one workgroup per work unit instead); 0 = automatic.  Results do not
+ * depend on it (bitwise). */
+int cp3_batch_set_tuning(cp3_batch* b, int cosine_block, int group_waves,
+                         int workgroups_per_cu);
 /* Times every candidate tuning on the current job list (reps launches each)
- * and keeps the fastest; reports it. */
+ * and keeps the fastest; reports it (any pointer may be NULL). */
 int cp3_batch_autotune(cp3_batch* b, int reps, int* cosine_block,
-                       int* waves_per_cu);
+                       int* group_waves, int* workgroups_per_cu);
 /* Capture the launch into a hipGraph on the next run and replay it after. */
 int cp3_batch_use_graph(cp3_batch* b, int enable);
 /* Enqueue the whole batch on the batch's stream; cp3_batch_wait waits. */

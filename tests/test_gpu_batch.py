@@ -54,7 +54,7 @@ def small_mix():
     return earth, grids, jobs, sets, beam_ev
 
 
-def build(precision, nc=0, waves=0):
+def build(precision, nc=0, group=0, waves=0):
     earth, grids, jobs, sets, beam_ev = small_mix()
     b = Batch(precision)
     gid = {k: b.add_atmos_grid(c, e, earth, 22.0) for k, (c, e) in grids.items()}
@@ -62,13 +62,14 @@ def build(precision, nc=0, waves=0):
     cj = [make_job(ATMOS, gid[j.grid], j.params, j.nutype) for j in jobs]
     cj += [make_job(BEAM, bg, sets[0], 0), make_job(BEAM, bg, sets[1], 1)]
     b.set_jobs(cj)
-    if nc or waves:
-        b.set_tuning(nc, waves)
+    if nc or group or waves:
+        b.set_tuning(nc, group, waves)
     return b, earth, jobs, sets, beam_ev
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("tuning", [(1, 0), (4, -1), (8, 2)])
+@pytest.mark.parametrize("tuning", [(1, 1, 0), (4, 1, -1), (8, 2, 2),
+                                    (2, 4, 0), (4, 4, -1)])
 def test_batch_equals_single(precision, tuning):
     b, earth, jobs, sets, beam_ev = build(precision, *tuning)
     b.run()
@@ -129,8 +130,8 @@ def test_update_jobs_and_fp64_parity(oracle):
 
 def test_autotune_picks_valid():
     b, *_ = build("mixed")
-    nc, waves = b.autotune(reps=1)
-    assert nc in (1, 2, 4, 8)
+    nc, group, waves = b.autotune(reps=1)
+    assert nc in (1, 2, 4, 8) and group in (1, 2, 4)
     b.run()
     b.wait()
     assert b.last_ms() > 0

@@ -19,11 +19,11 @@ TOL_FP32 = 2e-3
 
 
 def run_atmos(cz, ev, radii, rhos, yps, params, nutype, precision="fp64",
-              h_km=22.0, nc=0):
+              h_km=22.0, nc=0, w=0):
     from cudaprob3_amd import AtmosPropagator
     with AtmosPropagator(len(cz), len(ev), precision=precision) as p:
-        if nc:
-            p.setCosineBlock(nc)
+        if nc or w:
+            p.setTuning(nc, w)
         p.setEnergyList(ev)
         p.setCosineList(cz)
         t12, t13, t23, d, m12, m23 = [float(x) for x in params]
@@ -118,18 +118,17 @@ def test_beam_fp64_vs_golden():
 
 
 @pytest.mark.parametrize("nc", [1, 2, 4, 8])
-def test_cosine_blocks_identical(nc):
-    """The cosine-block size changes only which lanes share an eigen-solve,
-    never a point's arithmetic: results are bitwise identical."""
+@pytest.mark.parametrize("w", [1, 2, 4])
+def test_tunings_identical(nc, w):
+    """The cosine block and the waves sharing a solve change only which lanes
+    share an eigen-solve, never a point's arithmetic: bitwise identical.
+    37 cosines leave ragged last blocks (and idle waves) for every tuning."""
     e = W.earth_12()
     cz, ev = W.cos_grid(37), W.log_grid(70, 0.1, 100.0)
-    base = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, nc=1)
-    out = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, nc=nc)
-    assert np.array_equal(base, out)
-    for prec in ("fp32", "mixed"):
-        b = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=1)
-        o = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=nc)
-        assert np.array_equal(b, o)
+    for prec in ("fp64", "fp32", "mixed"):
+        b = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=1, w=1)
+        o = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=nc, w=w)
+        assert np.array_equal(b, o), prec
 
 
 @pytest.mark.parametrize("nutype", [0, 1])
