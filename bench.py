@@ -136,6 +136,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--sets", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -243,6 +244,15 @@ def main():
         result["mixed_err_vs_fp64_gpu"] = {"max": worst,
                                            "rms": float(np.sqrt(sq / n))}
         fp64.close()
+
+    if not args.no_serving:
+        # the request-serving workload: Poisson arrivals at 2000 req/s,
+        # latency submit -> result resident in HBM
+        from cudaprob3_amd.serving import run_stream
+        result["serving"] = {p: run_stream(p, 2000.0, 2.0)
+                             for p in ("mixed", "fp64")}
+        result["p50_us"] = result["serving"]["mixed"]["p50_us"]
+        result["p99_us"] = result["serving"]["mixed"]["p99_us"]
 
     if not args.no_cpu:
         cb, worst, rms = cpu_baseline(wl, mixed_results)

@@ -110,6 +110,21 @@ _SIGS.update({
     "cp3_batch_get_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
 })
 
+_SIGS.update({
+    "cp3_server_create": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "cp3_server_destroy": (C.c_int, [_vp]),
+    "cp3_server_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "cp3_server_add_atmos_grid": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, _ip]),
+    "cp3_server_add_beam_grid": (C.c_int, [_vp, _dp, C.c_int, C.c_double, C.c_double, _ip]),
+    "cp3_server_submit": (C.c_int, [_vp, C.POINTER(Cp3Job), _i64p]),
+    "cp3_server_wait": (C.c_int, [_vp, C.c_int64, C.c_int]),
+    "cp3_server_fetch": (C.c_int, [_vp, C.c_int64, _dp]),
+    "cp3_server_result_device": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp), _ip]),
+    "cp3_server_release": (C.c_int, [_vp, C.c_int64]),
+    "cp3_server_latency": (C.c_int, [_vp, C.c_int64, _dp, _dp]),
+    "cp3_server_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p]),
+})
+
 _lib = None
 
 

@@ -12,6 +12,7 @@
 
 #include "../../include/cp3_hip.h"
 #include "cp3_capi_util.hpp"
+#include "cp3_grid.hpp"
 #include "cp3_host.hpp"
 #include "cp3_launch.hpp"
 
@@ -24,36 +25,14 @@ using cp3::capi::PinnedBuf;
 
 namespace {
 
-// One resident atmospheric grid (trajectory table on the device).
-struct AtmosGrid {
-  int nC = 0, nE = 0, S = 0;
-  DevBuf ev, perm, M, Lvac, seg, rye;
-  cp3::GridDev dev() const {
-    cp3::GridDev g{};
-    g.ev = (const double*)ev.p;
-    g.perm = (const int32_t*)perm.p;
-    g.M = (const int32_t*)M.p;
-    g.Lvac = (const double*)Lvac.p;
-    g.seg = (const double*)seg.p;
-    g.rye = (const double*)rye.p;
-    g.nC = nC;
-    g.nE = nE;
-    g.S = S;
-    return g;
-  }
-};
-
-struct BeamGrid {
-  int nE = 0;
-  double rye = 0.0, L = 0.0;
-  DevBuf ev;
-};
-
 constexpr size_t kAlign = 256;
 
 size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 }  // namespace
+
+using cp3::capi::AtmosGrid;
+using cp3::capi::BeamGrid;
 
 struct cp3_batch {
   int dev = 0, precision = CP3_FP64;
@@ -418,22 +397,8 @@ int cp3_batch_add_atmos_grid(cp3_batch* b, const double* cosv, int nC,
       need_ptr(rhos, "rhos");
       need_ptr(yps, "yps");
     }
-    cp3::EarthModel em;
-    cp3::earth_set(em, std::vector<double>(radii, radii + nrows),
-                   std::vector<double>(rhos, rhos + nrows),
-                   std::vector<double>(yps, yps + nrows));
-    cp3::TrajTable tt;
-    cp3::build_traj_table(tt, em, cosv, nC, h_km * 100000.0);
-    std::unique_ptr<AtmosGrid> g(new AtmosGrid());
-    g->nC = nC;
-    g->nE = nE;
-    g->S = tt.S;
-    g->ev.upload(ev, (size_t)nE * sizeof(double), b->dev);
-    g->perm.upload(tt.perm.data(), tt.perm.size() * sizeof(int32_t), b->dev);
-    g->M.upload(tt.M.data(), tt.M.size() * sizeof(int32_t), b->dev);
-    g->Lvac.upload(tt.Lvac.data(), tt.Lvac.size() * sizeof(double), b->dev);
-    g->seg.upload(tt.seg.data(), tt.seg.size() * sizeof(double), b->dev);
-    g->rye.upload(tt.rye.data(), tt.rye.size() * sizeof(double), b->dev);
+    std::unique_ptr<AtmosGrid> g =
+        cp3::capi::make_atmos_grid(cosv, nC, ev, nE, radii, rhos, yps, nrows, h_km, b->dev);
     *grid_id = (int)b->agrids.size();
     b->agrids.push_back(std::move(g));
   });
@@ -446,11 +411,7 @@ int cp3_batch_add_beam_grid(cp3_batch* b, const double* ev, int nE, double rho,
     need_ptr(grid_id, "grid_id");
     if (nE <= 0) throw Error(CP3_ERR_INVALID, "number of energies must be positive");
     need_ptr(ev, "energies");
-    std::unique_ptr<BeamGrid> g(new BeamGrid());
-    g->nE = nE;
-    g->rye = rho * cp3::kBeamYe;
-    g->L = L;
-    g->ev.upload(ev, (size_t)nE * sizeof(double), b->dev);
+    std::unique_ptr<BeamGrid> g = cp3::capi::make_beam_grid(ev, nE, rho, L, b->dev);
     *grid_id = (int)b->bgrids.size();
     b->bgrids.push_back(std::move(g));
   });

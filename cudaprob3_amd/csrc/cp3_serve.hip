@@ -1,0 +1,571 @@
+// Request server (include/cp3_hip.h, cp3_server_*): continuous batching of
+// single-evaluation requests on one device.
+//
+// Clients submit jobs (a parameter set and neutrino type on a resident grid)
+// from any thread.  A scheduler thread keeps a FIFO of pending requests and
+// K launch slots (one stream each).  Whenever a slot is free it admits the
+// requests at the head of the queue whose outputs fit in the device-memory
+// arena (a first-fit allocator over a region of the configured cap), builds
+// their job table and launches the batched kernels on that slot's stream;
+// requests that arrive while other batches run are admitted into the next
+// free slot, so their work overlaps on the device.  A finished request's
+// output stays resident until the client fetches or releases it, which
+// returns its memory to the arena.
+//
+// Every tuning of the kernels gives bitwise identical results, so a request's
+// probabilities do not depend on which other requests share its batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cp3_hip.h"
+#include "cp3_capi_util.hpp"
+#include "cp3_grid.hpp"
+#include "cp3_host.hpp"
+#include "cp3_launch.hpp"
+
+using cp3::capi::AtmosGrid;
+using cp3::capi::BeamGrid;
+using cp3::capi::DevBuf;
+using cp3::capi::DeviceScope;
+using cp3::capi::Error;
+using cp3::capi::guard;
+using cp3::capi::hip_check;
+using cp3::capi::PinnedBuf;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator of offsets in [0, cap) with a coalescing free list.
+class ArenaAlloc {
+ public:
+  explicit ArenaAlloc(size_t cap) : cap_(cap) {
+    if (cap) free_[0] = cap;
+  }
+  static size_t round(size_t n) { return (n + 255) / 256 * 256; }
+  bool alloc(size_t n, size_t* off) {
+    n = round(n);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < n) continue;
+      *off = it->first;
+      const size_t rest = it->second - n, start = it->first + n;
+      free_.erase(it);
+      if (rest) free_[start] = rest;
+      used_ += n;
+      return true;
+    }
+    return false;
+  }
+  void release(size_t off, size_t n) {
+    n = round(n);
+    used_ -= n;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t used() const { return used_; }
+  size_t cap() const { return cap_; }
+
+ private:
+  std::map<size_t, size_t> free_;
+  size_t cap_, used_ = 0;
+};
+
+enum ReqState { kPending, kRunning, kDone, kFailed };
+
+struct Request {
+  cp3_job job{};
+  size_t bytes = 0, off = 0;
+  int64_t evals = 0;
+  Clock::time_point t_submit, t_launch, t_done;
+  ReqState state = kPending;
+  bool has_mem = false;
+  std::string err;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  std::vector<int64_t> reqs;
+  std::vector<cp3::AtmosJob> ajobs;
+  std::vector<cp3::BeamJob> bjobs;
+  DevBuf d_ajobs, d_bjobs;
+  PinnedBuf staging;
+};
+
+double us_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::micro>(b - a).count();
+}
+
+}  // namespace
+
+struct cp3_server {
+  int dev = 0, precision = CP3_MIXED;
+  int nc = 2, w = 4;
+  int max_batch = 256;
+  DevBuf arena;
+  std::unique_ptr<ArenaAlloc> alloc;
+  std::vector<std::unique_ptr<AtmosGrid>> agrids;
+  std::vector<std::unique_ptr<BeamGrid>> bgrids;
+  std::vector<std::unique_ptr<Slot>> slots;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<int64_t> pending;
+  std::unordered_map<int64_t, Request> reqs;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+  int64_t n_done = 0, evals_done = 0, n_batches = 0, peak_inflight = 0;
+  size_t peak_mem = 0;
+
+  ~cp3_server() { shutdown(); }
+
+  int elem() const { return cp3::out_elem_bytes(precision); }
+
+  void shutdown() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv_work.notify_all();
+    if (worker.joinable()) worker.join();
+    DeviceScope ds(dev);
+    for (auto& sp : slots) {
+      Slot& s = *sp;
+      if (s.stream) (void)hipStreamSynchronize(s.stream);
+      if (s.done) (void)hipEventDestroy(s.done);
+      if (s.stream) (void)hipStreamDestroy(s.stream);
+      s.done = nullptr;
+      s.stream = nullptr;
+    }
+  }
+
+  void check_job(const cp3_job& j) const {
+    if (j.kind == CP3_JOB_ATMOS) {
+      if (j.grid < 0 || (size_t)j.grid >= agrids.size())
+        throw Error(CP3_ERR_INVALID, "unknown atmospheric grid id " + std::to_string(j.grid));
+    } else if (j.kind == CP3_JOB_BEAM) {
+      if (j.grid < 0 || (size_t)j.grid >= bgrids.size())
+        throw Error(CP3_ERR_INVALID, "unknown beam grid id " + std::to_string(j.grid));
+    } else {
+      throw Error(CP3_ERR_INVALID, "job kind must be CP3_JOB_ATMOS or CP3_JOB_BEAM");
+    }
+    if (j.nutype != 0 && j.nutype != 1) throw Error(CP3_ERR_INVALID, "neutrino type must be 0 or 1");
+  }
+
+  size_t points(const cp3_job& j) const {
+    return j.kind == CP3_JOB_ATMOS ? agrids[(size_t)j.grid]->points()
+                                   : bgrids[(size_t)j.grid]->points();
+  }
+
+  // Called with mu held.  Returns false when the head of the queue does not
+  // fit in the arena (it waits for memory; admission stays FIFO).
+  bool launch_batch(Slot& s) {
+    std::vector<int64_t> take;
+    while (!pending.empty() && (int)take.size() < max_batch) {
+      Request& r = reqs[pending.front()];
+      size_t off = 0;
+      if (!alloc->alloc(r.bytes, &off)) break;
+      r.off = off;
+      r.has_mem = true;
+      take.push_back(pending.front());
+      pending.pop_front();
+    }
+    if (take.empty()) return false;
+    peak_mem = std::max(peak_mem, alloc->used());
+    s.ajobs.clear();
+    s.bjobs.clear();
+    int64_t au = 0, bu = 0;
+    for (int64_t id : take) {
+      const Request& r = reqs[id];
+      const cp3_job& in = r.job;
+      const cp3::SetConst sc = cp3::make_set(in.theta12, in.theta13, in.theta23, in.dcp,
+                                             in.knutype < 0 ? -1 : 1, in.dm12sq, in.dm23sq,
+                                             in.nutype);
+      void* out = (char*)arena.p + r.off;
+      if (in.kind == CP3_JOB_ATMOS) {
+        cp3::AtmosJob a{};
+        a.sc = sc;
+        a.g = agrids[(size_t)in.grid]->dev();
+        a.out = out;
+        a.unit_begin = au;
+        au += cp3::atmos_units(a.g, nc, w);
+        s.ajobs.push_back(a);
+      } else {
+        const BeamGrid& g = *bgrids[(size_t)in.grid];
+        cp3::BeamJob b{};
+        b.sc = sc;
+        b.ev = (const double*)g.ev.p;
+        b.nE = g.nE;
+        b.rye = g.rye;
+        b.L = g.L;
+        b.out = out;
+        b.unit_begin = bu;
+        bu += cp3::beam_units(g.nE);
+        s.bjobs.push_back(b);
+      }
+    }
+    try {
+      const size_t abytes = s.ajobs.size() * sizeof(cp3::AtmosJob);
+      const size_t bbytes = s.bjobs.size() * sizeof(cp3::BeamJob);
+      std::memcpy(s.staging.p, s.ajobs.data(), abytes);
+      std::memcpy((char*)s.staging.p + abytes, s.bjobs.data(), bbytes);
+      if (abytes)
+        hip_check(hipMemcpyAsync(s.d_ajobs.p, s.staging.p, abytes, hipMemcpyHostToDevice,
+                                 s.stream),
+                  "job table upload");
+      if (bbytes)
+        hip_check(hipMemcpyAsync(s.d_bjobs.p, (char*)s.staging.p + abytes, bbytes,
+                                 hipMemcpyHostToDevice, s.stream),
+                  "job table upload");
+      if (au > 0)
+        hip_check(cp3::launch_atmos_batch(precision, nc, w, (const cp3::AtmosJob*)s.d_ajobs.p,
+                                          (int)s.ajobs.size(), au,
+                                          (int)std::min<int64_t>(au, 0x7fffffff), s.stream),
+                  "atmospheric launch");
+      if (bu > 0)
+        hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)s.d_bjobs.p,
+                                         (int)s.bjobs.size(), bu,
+                                         (int)std::min<int64_t>(bu, 0x7fffffff), s.stream),
+                  "beam launch");
+      hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
+    } catch (const std::exception& e) {
+      for (int64_t id : take) fail(id, e.what());
+      cv_done.notify_all();
+      return true;
+    }
+    const Clock::time_point now = Clock::now();
+    for (int64_t id : take) {
+      Request& r = reqs[id];
+      r.state = kRunning;
+      r.t_launch = now;
+    }
+    s.reqs = std::move(take);
+    s.busy = true;
+    ++n_batches;
+    int64_t inflight = 0;
+    for (const auto& o : slots) inflight += o->busy ? 1 : 0;
+    peak_inflight = std::max(peak_inflight, inflight);
+    return true;
+  }
+
+  void fail(int64_t id, const std::string& why) {
+    Request& r = reqs[id];
+    r.state = kFailed;
+    r.err = why;
+    if (r.has_mem) {
+      alloc->release(r.off, r.bytes);
+      r.has_mem = false;
+    }
+  }
+
+  void retire(Slot& s, hipError_t status) {
+    const Clock::time_point now = Clock::now();
+    for (int64_t id : s.reqs) {
+      auto it = reqs.find(id);
+      if (it == reqs.end()) continue;
+      if (status == hipSuccess) {
+        it->second.state = kDone;
+        it->second.t_done = now;
+        ++n_done;
+        evals_done += it->second.evals;
+      } else {
+        fail(id, std::string("kernel failed: ") + hipGetErrorString(status));
+      }
+    }
+    s.reqs.clear();
+    s.busy = false;
+  }
+
+  void loop() {
+    DeviceScope ds(dev);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      bool progress = false, busy = false;
+      for (auto& sp : slots) {
+        Slot& s = *sp;
+        if (!s.busy) continue;
+        const hipError_t q = hipEventQuery(s.done);
+        if (q == hipErrorNotReady) {
+          busy = true;
+          continue;
+        }
+        retire(s, q);
+        progress = true;
+      }
+      if (progress) cv_done.notify_all();
+      for (auto& sp : slots) {
+        Slot& s = *sp;
+        if (pending.empty()) break;
+        if (s.busy) continue;
+        if (!launch_batch(s)) break;
+        busy = true;
+        progress = true;
+      }
+      if (stop && !busy) break;
+      if (progress) continue;
+      if (busy) {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(10));
+        lk.lock();
+      } else {
+        cv_work.wait(lk);
+      }
+    }
+    for (int64_t id : pending) fail(id, "server stopped");
+    pending.clear();
+    cv_done.notify_all();
+  }
+};
+
+namespace {
+
+cp3_server* need(cp3_server* s) {
+  if (!s) throw Error(CP3_ERR_INVALID, "null server handle");
+  return s;
+}
+
+void need_ptr(const void* p, const char* what) {
+  if (!p) throw Error(CP3_ERR_INVALID, std::string("null pointer: ") + what);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cp3_server_create(int device, int precision, size_t mem_cap_bytes, int n_slots,
+                      int max_batch_jobs, cp3_server** out) {
+  return guard([&] {
+    need_ptr(out, "out");
+    *out = nullptr;
+    if (precision != CP3_FP64 && precision != CP3_FP32 && precision != CP3_MIXED)
+      throw Error(CP3_ERR_INVALID, "unknown precision");
+    if (mem_cap_bytes == 0) throw Error(CP3_ERR_INVALID, "memory cap must be positive");
+    if (n_slots < 1 || n_slots > 16) throw Error(CP3_ERR_INVALID, "slots must be in 1..16");
+    if (max_batch_jobs < 1) throw Error(CP3_ERR_INVALID, "max batch must be positive");
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+      throw Error(CP3_ERR_INVALID,
+                  "The requested GPU Id " + std::to_string(device) + " is not available.");
+    std::unique_ptr<cp3_server> s(new cp3_server());
+    s->dev = device;
+    s->precision = precision;
+    s->max_batch = max_batch_jobs;
+    DeviceScope ds(device);
+    s->arena.alloc(mem_cap_bytes, device);
+    s->alloc.reset(new ArenaAlloc(mem_cap_bytes));
+    for (int k = 0; k < n_slots; ++k) s->slots.emplace_back(new Slot());
+    for (auto& slp : s->slots) {
+      Slot& sl = *slp;
+      hip_check(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate");
+      sl.d_ajobs.alloc((size_t)max_batch_jobs * sizeof(cp3::AtmosJob), device);
+      sl.d_bjobs.alloc((size_t)max_batch_jobs * sizeof(cp3::BeamJob), device);
+      sl.staging.alloc((size_t)max_batch_jobs *
+                       (sizeof(cp3::AtmosJob) + sizeof(cp3::BeamJob)));
+    }
+    cp3_server* raw = s.get();
+    s->worker = std::thread([raw] { raw->loop(); });
+    *out = s.release();
+  });
+}
+
+int cp3_server_destroy(cp3_server* s) {
+  return guard([&] { delete s; });
+}
+
+int cp3_server_set_tuning(cp3_server* s, int nc, int w) {
+  return guard([&] {
+    need(s);
+    if (!cp3::valid_nc(nc)) throw Error(CP3_ERR_INVALID, "cosine block must be 1, 2, 4 or 8");
+    if (!cp3::valid_group(w)) throw Error(CP3_ERR_INVALID, "group waves must be 1, 2 or 4");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->nc = nc;
+    s->w = w;
+  });
+}
+
+int cp3_server_add_atmos_grid(cp3_server* s, const double* cosv, int nC, const double* ev,
+                              int nE, const double* radii, const double* rhos,
+                              const double* yps, int nrows, double h_km, int* grid_id) {
+  return guard([&] {
+    need(s);
+    need_ptr(grid_id, "grid_id");
+    std::unique_ptr<AtmosGrid> g =
+        cp3::capi::make_atmos_grid(cosv, nC, ev, nE, radii, rhos, yps, nrows, h_km, s->dev);
+    std::lock_guard<std::mutex> lk(s->mu);
+    *grid_id = (int)s->agrids.size();
+    s->agrids.push_back(std::move(g));
+  });
+}
+
+int cp3_server_add_beam_grid(cp3_server* s, const double* ev, int nE, double rho,
+                             double L, int* grid_id) {
+  return guard([&] {
+    need(s);
+    need_ptr(grid_id, "grid_id");
+    std::unique_ptr<BeamGrid> g = cp3::capi::make_beam_grid(ev, nE, rho, L, s->dev);
+    std::lock_guard<std::mutex> lk(s->mu);
+    *grid_id = (int)s->bgrids.size();
+    s->bgrids.push_back(std::move(g));
+  });
+}
+
+int cp3_server_submit(cp3_server* s, const cp3_job* job, int64_t* request_id) {
+  return guard([&] {
+    need(s);
+    need_ptr(job, "job");
+    need_ptr(request_id, "request_id");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) throw Error(CP3_ERR_STATE, "server is stopping");
+    s->check_job(*job);
+    Request r;
+    r.job = *job;
+    r.evals = (int64_t)s->points(*job);
+    r.bytes = (size_t)r.evals * 9 * (size_t)s->elem();
+    if (ArenaAlloc::round(r.bytes) > s->alloc->cap())
+      throw Error(CP3_ERR_CAPACITY, "request output (" + std::to_string(r.bytes) +
+                                        " B) exceeds the server's memory cap");
+    r.t_submit = Clock::now();
+    const int64_t id = s->next_id++;
+    s->reqs.emplace(id, std::move(r));
+    s->pending.push_back(id);
+    *request_id = id;
+    s->cv_work.notify_one();
+  });
+}
+
+int cp3_server_wait(cp3_server* s, int64_t id, int timeout_ms) {
+  return guard([&] {
+    need(s);
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] {
+      auto it = s->reqs.find(id);
+      return it == s->reqs.end() || it->second.state == kDone || it->second.state == kFailed;
+    };
+    if (timeout_ms < 0) {
+      s->cv_done.wait(lk, ready);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+      throw Error(CP3_ERR_STATE, "timed out waiting for request " + std::to_string(id));
+    }
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) throw Error(CP3_ERR_RANGE, "unknown request " + std::to_string(id));
+    if (it->second.state == kFailed) throw Error(CP3_ERR_DEVICE, it->second.err);
+  });
+}
+
+int cp3_server_fetch(cp3_server* s, int64_t id, double* dst) {
+  return guard([&] {
+    need(s);
+    need_ptr(dst, "dst");
+    size_t off, bytes;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->reqs.find(id);
+      if (it == s->reqs.end()) throw Error(CP3_ERR_RANGE, "unknown request " + std::to_string(id));
+      if (it->second.state != kDone) throw Error(CP3_ERR_STATE, "request is not finished");
+      off = it->second.off;
+      bytes = it->second.bytes;
+    }
+    {
+      DeviceScope ds(s->dev);
+      if (s->precision == CP3_FP64) {
+        hip_check(hipMemcpy(dst, (const char*)s->arena.p + off, bytes, hipMemcpyDeviceToHost),
+                  "result copy");
+      } else {
+        std::vector<float> tmp(bytes / sizeof(float));
+        hip_check(hipMemcpy(tmp.data(), (const char*)s->arena.p + off, bytes,
+                            hipMemcpyDeviceToHost),
+                  "result copy");
+        for (size_t i = 0; i < tmp.size(); ++i) dst[i] = (double)tmp[i];
+      }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it != s->reqs.end()) {
+      if (it->second.has_mem) s->alloc->release(it->second.off, it->second.bytes);
+      s->reqs.erase(it);
+    }
+    s->cv_work.notify_one();
+  });
+}
+
+int cp3_server_result_device(cp3_server* s, int64_t id, const void** dptr, int* elem) {
+  return guard([&] {
+    need(s);
+    need_ptr(dptr, "dptr");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) throw Error(CP3_ERR_RANGE, "unknown request " + std::to_string(id));
+    if (it->second.state != kDone) throw Error(CP3_ERR_STATE, "request is not finished");
+    *dptr = (const char*)s->arena.p + it->second.off;
+    if (elem) *elem = s->elem();
+  });
+}
+
+int cp3_server_release(cp3_server* s, int64_t id) {
+  return guard([&] {
+    need(s);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) throw Error(CP3_ERR_RANGE, "unknown request " + std::to_string(id));
+    if (it->second.state == kPending || it->second.state == kRunning)
+      throw Error(CP3_ERR_STATE, "request is not finished");
+    if (it->second.has_mem) s->alloc->release(it->second.off, it->second.bytes);
+    s->reqs.erase(it);
+    s->cv_work.notify_one();
+  });
+}
+
+int cp3_server_latency(cp3_server* s, int64_t id, double* queue_us, double* total_us) {
+  return guard([&] {
+    need(s);
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) throw Error(CP3_ERR_RANGE, "unknown request " + std::to_string(id));
+    if (it->second.state != kDone) throw Error(CP3_ERR_STATE, "request is not finished");
+    if (queue_us) *queue_us = us_between(it->second.t_submit, it->second.t_launch);
+    if (total_us) *total_us = us_between(it->second.t_submit, it->second.t_done);
+  });
+}
+
+int cp3_server_stats(cp3_server* s, int64_t* completed, int64_t* evals, int64_t* batches,
+                     int64_t* mem_in_use, int64_t* mem_peak, int64_t* peak_inflight) {
+  return guard([&] {
+    need(s);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (completed) *completed = s->n_done;
+    if (evals) *evals = s->evals_done;
+    if (batches) *batches = s->n_batches;
+    if (mem_in_use) *mem_in_use = (int64_t)s->alloc->used();
+    if (mem_peak) *mem_peak = (int64_t)s->peak_mem;
+    if (peak_inflight) *peak_inflight = s->peak_inflight;
+  });
+}
+
+}  // extern "C"

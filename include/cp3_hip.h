@@ -230,6 +230,54 @@ int cp3_batch_stats(cp3_batch* b, int64_t* units, int64_t* evals,
                     int64_t* out_bytes);
 int cp3_batch_get_stream(cp3_batch* b, void** stream);
 
+
+/* ---------------------------------------------------------------------------
+ * Request server: continuous batching of single evaluations (fit or MCMC
+ * requests) on one device.  No reference counterpart.
+ *
+ * A scheduler thread admits pending requests, in arrival order, into free
+ * launch slots (one stream each) as long as their outputs fit in a device
+ * arena of mem_cap_bytes; batches on different slots overlap on the device.
+ * Outputs stay resident until fetched or released.  A request's result does
+ * not depend on the other requests of its batch (bitwise).  All functions
+ * are thread-safe.
+ * ------------------------------------------------------------------------- */
+
+typedef struct cp3_server cp3_server;
+
+int cp3_server_create(int device, int precision, size_t mem_cap_bytes,
+                      int n_slots, int max_batch_jobs, cp3_server** out);
+/* Drains admitted work, fails the rest, frees everything. */
+int cp3_server_destroy(cp3_server* s);
+/* Kernel tuning for every batch (cosine block, group waves). */
+int cp3_server_set_tuning(cp3_server* s, int cosine_block, int group_waves);
+int cp3_server_add_atmos_grid(cp3_server* s, const double* cosines,
+                              int n_cosines, const double* energies,
+                              int n_energies, const double* radii_km,
+                              const double* rhos, const double* yps,
+                              int n_rows, double height_km, int* grid_id);
+int cp3_server_add_beam_grid(cp3_server* s, const double* energies,
+                             int n_energies, double rho, double length_km,
+                             int* grid_id);
+/* Queues one job; CP3_ERR_CAPACITY if its output can never fit the cap. */
+int cp3_server_submit(cp3_server* s, const cp3_job* job, int64_t* request_id);
+/* Waits until the request finished (timeout_ms < 0: no limit). */
+int cp3_server_wait(cp3_server* s, int64_t request_id, int timeout_ms);
+/* Copies a finished request's result to host as double (device layout) and
+ * releases it. */
+int cp3_server_fetch(cp3_server* s, int64_t request_id, double* host_dst);
+/* Device pointer of a finished request's result (valid until released). */
+int cp3_server_result_device(cp3_server* s, int64_t request_id,
+                             const void** dptr, int* elem_bytes);
+/* Releases a finished request's output memory without copying. */
+int cp3_server_release(cp3_server* s, int64_t request_id);
+/* Microseconds from submit to launch and from submit to completion. */
+int cp3_server_latency(cp3_server* s, int64_t request_id, double* queue_us,
+                       double* total_us);
+int cp3_server_stats(cp3_server* s, int64_t* completed, int64_t* evals,
+                     int64_t* batches, int64_t* mem_in_use, int64_t* mem_peak,
+                     int64_t* peak_inflight_batches);
+
 #ifdef __cplusplus
 }
 #endif
