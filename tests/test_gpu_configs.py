@@ -1,0 +1,88 @@
+"""Per-configuration measurements against the FP64 oracle (the reference
+CPU path restated; oracle/_ref agrees with it bitwise):
+
+* the 12-shell precision study: 2000 x 2000 (cos x E, 0.1-100 GeV), nominal
+  parameters, neutrino and antineutrino, in plain FP64, plain FP32 and mixed;
+* the beam batch: 4096 seeded parameter sets x 1e5 energies (0.1-20 GeV,
+  1300 km, 2.6 g/cm^3) in one mixed-precision launch, checked on sampled sets.
+
+Each prints max / RMS absolute error and evals/s; the bounds asserted are the
+stated tolerances (FP64 1e-10, mixed 1e-5).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import osc_for
+from cudaprob3_amd import workloads as W
+from cudaprob3_amd.batch import BEAM, Batch, make_job
+
+pytestmark = pytest.mark.gpu
+
+NOM = [W.TH12, W.TH13, W.TH23, W.DCP, W.DM12, W.DM23]
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_precision_study_shells12_2000(oracle, nutype):
+    from cudaprob3_amd import AtmosPropagator
+    e = W.earth_12()
+    cz, ev = W.cos_grid(2000), W.log_grid(2000, 0.1, 100.0)
+    t0 = time.perf_counter()
+    ref = oracle.atmos(nutype, cz, ev, e.radii, e.rhos, e.yps,
+                       osc_for(NOM, nutype), 22.0, threads=16)
+    t_cpu = time.perf_counter() - t0
+    report = {}
+    for prec in ("fp64", "fp32", "mixed"):
+        with AtmosPropagator(2000, 2000, precision=prec) as p:
+            p.setEnergyList(ev)
+            p.setCosineList(cz)
+            t12, t13, t23, d, m12, m23 = NOM
+            p.setMNSMatrix(t12, t13, t23, d, -1 if nutype else 1)
+            p.setNeutrinoMasses(m12, m23)
+            p.setDensity(e.radii, e.rhos, e.yps)
+            p.setProductionHeight(22.0)
+            ms = []
+            for _ in range(4):
+                p.calculateProbabilities(nutype)
+                ms.append(p.lastKernelMs())
+            out = p.probabilities()
+        d = np.abs(out - ref)
+        report[prec] = (float(d.max()), float(np.sqrt((d ** 2).mean())),
+                        4e6 / (np.median(ms) / 1e3))
+    for prec, (mx, rms, rate) in report.items():
+        print(f"shells12 2000x2000 nutype {nutype} {prec}: max {mx:.2e} "
+              f"rms {rms:.2e}  {rate:.3e} evals/s")
+    print(f"  oracle (16 threads): {4e6 / t_cpu:.3e} evals/s")
+    assert report["fp64"][0] < 1e-10
+    assert report["mixed"][0] < 1e-5
+
+
+def test_beam_batch_4096x1e5_one_launch(oracle):
+    ev = W.log_grid(100000, 0.1, 20.0)
+    sets = W.draw_param_sets(4096, 4096)
+    with Batch("mixed") as b:
+        g = b.add_beam_grid(ev, 2.6, 1300.0)
+        b.set_jobs([make_job(BEAM, g, p, 0) for p in sets])
+        b.use_graph(True)
+        b.run()
+        b.wait()
+        ms = []
+        for _ in range(5):
+            b.run()
+            b.wait()
+            ms.append(b.last_ms())
+        rate = 4096 * 1e5 / (np.median(ms) / 1e3)
+        worst, sq, n = 0.0, 0.0, 0
+        for i in range(0, 4096, 256):
+            p = sets[i]
+            row = [p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+            ref = oracle.beam(0, ev, 2.6, 1300.0, osc_for(row, 0), threads=16)
+            d = np.abs(b.result(i).T - ref)
+            worst = max(worst, float(d.max()))
+            sq += float((d ** 2).sum())
+            n += d.size
+    print(f"beam 4096 sets x 1e5 E, one mixed launch: {np.median(ms):.3f} ms, "
+          f"{rate:.3e} evals/s; sampled 16 sets: max {worst:.2e} "
+          f"rms {np.sqrt(sq / n):.2e}")
+    assert worst < 1e-5
