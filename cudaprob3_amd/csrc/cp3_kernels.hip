@@ -28,27 +28,42 @@ struct OutT<PolicyF64> {
   using type = double;
 };
 
-// Applies shell i's slab to the NC trajectories of a wave.  Mmin is the
-// smallest shell count of the wave's trajectories: shells below Mmin - 1 are
-// middle shells of all of them and run branch-free, so the NC independent
-// update chains interleave.
+// Reads of a grid's tables (perm, M, seg, rye, Lvac) at wave-uniform indices.
+// No kernel writes them, so they are read through the constant address
+// space: scalar loads into SGPRs, with the index arithmetic on the scalar unit
+// instead of per-lane 64-bit address math.
+template <class T>
+__device__ __forceinline__ T uload(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+
+// Shell i as a middle shell of all NC trajectories of a wave (i < Mmin - 1,
+// Mmin the smallest shell count among them): branch-free, so the NC
+// independent update chains interleave.
+template <class P, int NC>
+__device__ __forceinline__ void middle_all(Traj<P> (&t)[NC], const Slab<P>& s,
+                                           const GridDev& g, const int (&cidx)[NC],
+                                           int i) {
+  using D = typename P::D;
+#pragma unroll
+  for (int j = 0; j < NC; ++j)
+    traj_middle<P>(t[j], s, (D)uload(g.seg, (int64_t)cidx[j] * g.S + i));
+}
+
+// Shell i for trajectories that may end at or before it: a middle shell of
+// those with more shells, the innermost of those with exactly i + 1.
 template <class P, int NC>
 __device__ __forceinline__ void apply_shell(Traj<P> (&t)[NC], const Slab<P>& s,
                                             const GridDev& g, const int (&cidx)[NC],
-                                            const int (&Mc)[NC], int Mmin, int i) {
+                                            const int (&Mc)[NC], int i) {
   using D = typename P::D;
-  if (i < Mmin - 1) {
 #pragma unroll
-    for (int j = 0; j < NC; ++j)
-      traj_middle<P>(t[j], s, (D)g.seg[(int64_t)cidx[j] * g.S + i]);
-  } else {
-#pragma unroll
-    for (int j = 0; j < NC; ++j) {
-      if (i < Mc[j] - 1) {
-        traj_middle<P>(t[j], s, (D)g.seg[(int64_t)cidx[j] * g.S + i]);
-      } else if (i == Mc[j] - 1) {
-        traj_inner<P>(t[j], s, (D)g.seg[(int64_t)cidx[j] * g.S + i]);
-      }
+  for (int j = 0; j < NC; ++j) {
+    const D L = (D)uload(g.seg, (int64_t)cidx[j] * g.S + i);
+    if (i < Mc[j] - 1) {
+      traj_middle<P>(t[j], s, L);
+    } else if (i == Mc[j] - 1) {
+      traj_inner<P>(t[j], s, L);
     }
   }
 }
@@ -88,6 +103,22 @@ __device__ __forceinline__ void stage_get(const SlabStage<P, W>& st, int buf,
 // Dummy staging type for W == 1 (no LDS).
 struct NoStage {};
 
+// Wave `wave`'s share of chunk c: the solve of shell c * W + wave, if the
+// unit crosses it, published in LDS buffer buf.
+template <class P, int W>
+__device__ __forceinline__ void stage_chunk(SlabStage<P, W>& st, int buf, int c,
+                                            int wave, int Mwg, const SetConst& sc,
+                                            typename P::D E, typename P::D inv,
+                                            const GridDev& g, int lane) {
+  using D = typename P::D;
+  const int is = c * W + wave;
+  if (is < Mwg) {
+    Slab<P> s;
+    solve_slab<P>(sc, E, inv, (D)uload(g.rye, is), s);
+    stage_put<P, W>(st, buf, wave, lane, s);
+  }
+}
+
 template <class P, int NC, int W, class Stage>
 __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
                                            Stage& st) {
@@ -114,8 +145,8 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
   for (int j = 0; j < NC; ++j) {
     const int k = base + wave * NC + j;
     c_ok[j] = k < g.nC;
-    cidx[j] = g.perm[c_ok[j] ? k : g.nC - 1];
-    Mc[j] = c_ok[j] ? g.M[cidx[j]] : 0;
+    cidx[j] = uload(g.perm, c_ok[j] ? k : g.nC - 1);
+    Mc[j] = c_ok[j] ? uload(g.M, cidx[j]) : 0;
     Mmax = Mc[j] > Mmax ? Mc[j] : Mmax;
     Mmin = Mc[j] < Mmin ? Mc[j] : Mmin;
   }
@@ -124,32 +155,53 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
 #pragma unroll
   for (int j = 0; j < NC; ++j) traj_init<P>(t[j], Mc[j] == 0);
 
+  // Shells below Mmin - 1 run in a loop of their own: joining the
+  // branch-free path with the general one inside a single loop costs
+  // register copies of the whole trajectory state at every shell.
   if constexpr (W == 1) {
     (void)st;
-    for (int i = 0; i < Mmax; ++i) {
+    int i = 0;
+    for (; i < Mmin - 1; ++i) {
       Slab<P> s;
-      solve_slab<P>(job.sc, E, inv, (D)g.rye[i], s);
-      apply_shell<P, NC>(t, s, g, cidx, Mc, Mmin, i);
+      solve_slab<P>(job.sc, E, inv, (D)uload(g.rye, i), s);
+      middle_all<P, NC>(t, s, g, cidx, i);
+    }
+    for (; i < Mmax; ++i) {
+      Slab<P> s;
+      solve_slab<P>(job.sc, E, inv, (D)uload(g.rye, i), s);
+      apply_shell<P, NC>(t, s, g, cidx, Mc, i);
     }
   } else {
     // deepest trajectory of the unit: its first cosine (decreasing order)
-    const int Mwg = g.M[g.perm[base]];
+    const int Mwg = uload(g.M, uload(g.perm, base));
+    const int nchunk = (Mwg + W - 1) / W;
+    // chunks of W shells that are middle shells of every trajectory of this
+    // wave; nfast < nchunk since Mmin <= Mwg.  Every wave passes nchunk
+    // barriers whatever its own split.
+    const int nfast = Mmin > 1 ? (Mmin - 1) / W : 0;
     int buf = 0;
-    for (int i0 = 0; i0 < Mwg; i0 += W) {
-      const int is = i0 + wave;
-      if (is < Mwg) {
-        Slab<P> s;
-        solve_slab<P>(job.sc, E, inv, (D)g.rye[is], s);
-        stage_put<P, W>(st, buf, wave, lane, s);
-      }
+    int c = 0;
+    for (; c < nfast; ++c) {
+      stage_chunk<P, W>(st, buf, c, wave, Mwg, job.sc, E, inv, g, lane);
       __syncthreads();
 #pragma unroll
       for (int w = 0; w < W; ++w) {
-        const int i = i0 + w;
+        Slab<P> s;
+        stage_get<P, W>(st, buf, w, lane, s);
+        middle_all<P, NC>(t, s, g, cidx, c * W + w);
+      }
+      buf ^= 1;
+    }
+    for (; c < nchunk; ++c) {
+      stage_chunk<P, W>(st, buf, c, wave, Mwg, job.sc, E, inv, g, lane);
+      __syncthreads();
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const int i = c * W + w;
         if (i < Mmax) {
           Slab<P> s;
           stage_get<P, W>(st, buf, w, lane, s);
-          apply_shell<P, NC>(t, s, g, cidx, Mc, Mmin, i);
+          apply_shell<P, NC>(t, s, g, cidx, Mc, i);
         }
       }
       buf ^= 1;
@@ -166,7 +218,7 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     F prob[9];
-    traj_finish<P>(t[j], vac, (D)g.Lvac[cidx[j]], job.sc, prob);
+    traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cidx[j]), job.sc, prob);
     if (e_ok && c_ok[j]) {
       O* dst = out + (int64_t)cidx[j] * g.nE + e;
 #pragma unroll

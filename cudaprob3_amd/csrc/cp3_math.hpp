@@ -33,9 +33,9 @@
 //      det(Y) conj(y_0 x y_1), with det(Y) = e^{i sum psi} accumulated.
 //   4. cos(t0 -+ 2pi/3) = -cos(t0)/2 +- (sqrt(3)/2) sin(t0): one sincos for
 //      the three roots.
-// Phases: FP64 sincos is an inline Cody-Waite reduction plus Taylor series;
-// the FP32 and mixed forms reduce the phase to quarter revolutions (in FP64
-// for the mixed form) and evaluate FP32 Taylor series of sin and cos.
+// Phases: every layer phase is carried in revolutions, reduced to
+// [-1/2, 1/2] in P::D, and e^{i psi} - 1 is evaluated in P::F from minimax
+// polynomials of sin and cos of the half angle (no quadrant logic).
 #pragma once
 
 #if defined(__HIPCC__)
@@ -155,91 +155,89 @@ CP3_HD void m_sincos(float x, float* s, float* c) {
 }
 #endif
 
-CP3_HD int m_toint(double x) { return (int)(int64_t)x; }
-CP3_HD int m_toint(float x) { return (int)x; }
-
-// sin and cos of (x + n) * pi/2 for |x| <= 1/2 in FP32: the Taylor series of
-// sin through t^9 and of cos through t^8 on t = x*pi/2 in [-pi/4, pi/4]
-// (truncation below 2e-8), then the quadrant n mod 4.
-CP3_HD void m_sincos_quarter(float x, int n, float* s, float* c) {
-  const float t = x * 1.5707963267948966f;
-  const float z = t * t;
-  float ps = 2.7557319223985893e-06f;
-  ps = ps * z + -0.0001984126984126984f;
-  ps = ps * z + 0.008333333333333333f;
-  ps = ps * z + -0.16666666666666666f;
-  const float sr = (t * z) * ps + t;
-  float pc = 2.48015873015873e-05f;
-  pc = pc * z + -0.001388888888888889f;
-  pc = pc * z + 0.041666666666666664f;
-  pc = pc * z + -0.5f;
-  const float cr = z * pc + 1.0f;
-  const float so = (n & 1) ? cr : sr;
-  const float co = (n & 1) ? sr : cr;
-  *s = (n & 2) ? -so : so;
-  *c = ((n + 1) & 2) ? -co : co;
+// w = e^{2 pi i f} - 1 for |f| <= 1/2, with no argument reduction and no
+// quadrant: a = sqrt(2) sin(pi f) and b = sqrt(2) cos(pi f) from minimax
+// polynomials in f (degrees 9 and 10 in FP32, 17 and 16 in FP64; the
+// coefficients and their errors, at most 2.1e-7 and 4.5e-16 with rounding,
+// come from fit_cispi.py), then w = (cos 2 pi f - 1, sin 2 pi f)
+// = (-2 sin^2, 2 sin cos)(pi f) = (-a^2, a b), free of cancellation for
+// small f.
+CP3_HD void m_cispi_m1(float f, float* wr, float* wi) {
+  const float z = f * f;
+  float pa = 0.10920575261116028f;
+  pa = pa * z + -0.845763623714447f;
+  pa = pa * z + 3.6062889099121094f;
+  pa = pa * z + -7.308241367340088f;
+  pa = pa * z + 4.442883014678955f;
+  const float a = pa * f;
+  float b = -0.034502167254686356f;
+  b = b * z + 0.3322518765926361f;
+  b = b * z + -1.8882750272750854f;
+  b = b * z + 5.73988151550293f;
+  b = b * z + -6.9788641929626465f;
+  b = b * z + 1.4142135381698608f;
+  *wr = -(a * a);
+  *wi = a * b;
+}
+CP3_HD void m_cispi_m1(double f, double* wr, double* wi) {
+  const double z = f * f;
+  double pa = 1.0866733025717579e-06;
+  pa = fma(pa, z, -3.097413590743419e-05);
+  pa = fma(pa, z, 0.0006594466111570788);
+  pa = fma(pa, z, -0.01042336257001818);
+  pa = fma(pa, z, 0.11617182685681435);
+  pa = fma(pa, z, -0.8474880248098783);
+  pa = fma(pa, z, 3.606476571470557);
+  pa = fma(pa, z, -7.308249499995437);
+  pa = fma(pa, z, 4.442882938158366);
+  const double a = pa * f;
+  double b = 5.8684089087724815e-06;
+  b = fma(b, z, -0.00014787947669297268);
+  b = fma(b, z, 0.0027288046656365815);
+  b = fma(b, z, -0.03649645205784972);
+  b = fma(b, z, 0.33280776877138185);
+  b = fma(b, z, -1.8883467170302788);
+  b = fma(b, z, 5.739885734946355);
+  b = fma(b, z, -6.978864199638876);
+  b = fma(b, z, 1.4142135623730951);
+  *wr = -(a * a);
+  *wi = a * b;
 }
 
-// e^{i phase} - 1 for a layer, with the phase added to the determinant
-// accumulator of the trajectory (acc, quad).
-//   PhaseRad (FP64): om in radians per km; phase = om * L; acc in radians.
-//   PhaseQuarter (FP32, mixed): om in quarter revolutions per km; the phase
-//   q = om * L is reduced in D to k = rint(q), x = q - k, quadrant k mod 4,
-//   and evaluated in FP32; acc keeps the fractions, quad the quadrants.
-template <class D>
-struct PhaseRad {
-  static constexpr double kOmScale = 1.0;
-  CP3_HD static void eval(D om, D L, D& acc, int& quad, D& wr, D& wi) {
-    (void)quad;
-    const D ph = om * L;
-    acc += ph;
-    D s, c;
-    m_sincos(ph, &s, &c);
-    wr = c - D(1);
-    wi = s;
-  }
-  CP3_HD static void unit(D acc, int quad, D& cr, D& ci) {
-    (void)quad;
-    m_sincos(acc, &ci, &cr);
-  }
-};
-
+// Layer phases in revolutions: om is in revolutions per km, the phase
+// q = om * L is reduced in D to f = q - rint(q) in [-1/2, 1/2], summed into
+// acc (det Y = e^{2 pi i acc}), and e^{2 pi i f} - 1 is evaluated in F.
 template <class D, class F>
-struct PhaseQuarter {
-  static constexpr double kOmScale = 0.63661977236758134308;  // 2/pi
-  CP3_HD static void eval(D om, D L, D& acc, int& quad, F& wr, F& wi) {
+struct PhaseRev {
+  static constexpr double kOmScale = kInvTwoPi;
+  CP3_HD static void eval(D om, D L, D& acc, F& wr, F& wi) {
     const D q = om * L;
-    const D k = m_rint(q);
-    const D x = q - k;
-    acc += x;
-    const int n = m_toint(k);
-    quad += n;
-    F s, c;
-    m_sincos_quarter(F(x), n, &s, &c);
-    wr = c - F(1);
-    wi = s;
+    const D f = q - m_rint(q);
+    acc += f;
+    // an FP32 sum is kept in [-1/2, 1/2] so that it keeps its resolution
+    if (sizeof(D) == 4) acc -= m_rint(acc);
+    m_cispi_m1(F(f), &wr, &wi);
   }
-  CP3_HD static void unit(D acc, int quad, F& cr, F& ci) {
-    const D k = m_rint(acc);
-    const D x = acc - k;
-    m_sincos_quarter(F(x), quad + m_toint(k), &ci, &cr);
+  // the same without accumulating the phase
+  CP3_HD static void eval(D om, D L, F& wr, F& wi) {
+    const D q = om * L;
+    m_cispi_m1(F(q - m_rint(q)), &wr, &wi);
+  }
+  // e^{2 pi i acc}
+  CP3_HD static void unit(D acc, F& cr, F& ci) {
+    F wr;
+    m_cispi_m1(F(acc - m_rint(acc)), &wr, &ci);
+    cr = wr + F(1);
   }
 };
 
 template <class P>
-struct Phase;
-template <>
-struct Phase<PolicyF64> : PhaseRad<double> {};
-template <>
-struct Phase<PolicyF32> : PhaseQuarter<float, float> {};
-template <>
-struct Phase<PolicyMixed> : PhaseQuarter<double, float> {};
+struct Phase : PhaseRev<typename P::D, typename P::F> {};
 
 // ------------------------------------------------------- per (E, shell)
 
 // Eigen-solution of one homogeneous slab: two unit eigenvectors and their
-// phase rates relative to the reference eigenvalue (radians per km, or
-// revolutions per km for the mixed policy).
+// phase rates relative to the reference eigenvalue (revolutions per km).
 template <class P>
 struct Slab {
   typename P::F va[3], vb[3];
@@ -379,7 +377,6 @@ template <class P>
 struct Traj {
   C3<typename P::F> y0, y1;
   typename P::D acc;
-  int quad;
 };
 
 template <class P>
@@ -402,7 +399,6 @@ CP3_HD void traj_init(Traj<P>& t, bool no_shells) {
     t.y1.im[m] = zero;
   }
   t.acc = typename P::D(0);
-  t.quad = 0;
 }
 
 // A middle shell crossed twice (physics.hpp:971-978): Y -> T Y.
@@ -410,10 +406,10 @@ template <class P>
 CP3_HD void traj_middle(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   using F = typename P::F;
   F wr, wi;
-  Phase<P>::eval(s.oma, L, t.acc, t.quad, wr, wi);
+  Phase<P>::eval(s.oma, L, t.acc, wr, wi);
   rank1(s.va, wr, wi, t.y0);
   rank1(s.va, wr, wi, t.y1);
-  Phase<P>::eval(s.omb, L, t.acc, t.quad, wr, wi);
+  Phase<P>::eval(s.omb, L, t.acc, wr, wi);
   rank1(s.vb, wr, wi, t.y0);
   rank1(s.vb, wr, wi, t.y1);
 }
@@ -439,10 +435,9 @@ CP3_HD void dot_cr(const C3<F>& p, const F v[3], F& r, F& i) {
 template <class P>
 CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   using F = typename P::F;
-  using D = typename P::D;
   // third column y2 = det(Y) conj(y0 x y1)
   F er, ei;
-  Phase<P>::unit(t.acc, t.quad, er, ei);
+  Phase<P>::unit(t.acc, er, ei);
   C3<F> y2;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
@@ -466,13 +461,11 @@ CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   dot_cc(t.y1, t.y1, zr[3], zi[3]);
   dot_cc(t.y1, y2, zr[4], zi[4]);
   dot_cc(y2, y2, zr[5], zi[5]);
-  D dummy = D(0);
-  int dq = 0;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const F* v = k == 0 ? s.va : s.vb;
     F wr, wi;
-    Phase<P>::eval(k == 0 ? s.oma : s.omb, L, dummy, dq, wr, wi);
+    Phase<P>::eval(k == 0 ? s.oma : s.omb, L, wr, wi);
     F ur[3], ui[3], sr[3], si[3];
     dot_cr(t.y0, v, ur[0], ui[0]);
     dot_cr(t.y1, v, ur[1], ui[1]);
@@ -505,7 +498,6 @@ CP3_HD void traj_finish(const Traj<P>& t, const Slab<P>& vac,
                         typename P::D Lvac, const SetConst& sc,
                         typename P::F prob[9]) {
   using F = typename P::F;
-  using D = typename P::D;
   // A = Z T_0 = Z + sum_k w_k (Z v_k) v_k^T
   F ar[3][3], ai[3][3];
   const F zr[3][3] = {{t.y0.re[0], t.y0.re[1], t.y0.re[2]},
@@ -519,12 +511,10 @@ CP3_HD void traj_finish(const Traj<P>& t, const Slab<P>& vac,
       ar[i][j] = zr[i][j];
       ai[i][j] = zi[i][j];
     }
-  D dummy = D(0);
-  int dq = 0;
   for (int k = 0; k < 2; ++k) {
     const F* v = k == 0 ? vac.va : vac.vb;
     F wr, wi;
-    Phase<P>::eval(k == 0 ? vac.oma : vac.omb, Lvac, dummy, dq, wr, wi);
+    Phase<P>::eval(k == 0 ? vac.oma : vac.omb, Lvac, wr, wi);
     for (int i = 0; i < 3; ++i) {
       const F tr = zr[i][0] * v[0] + zr[i][1] * v[1] + zr[i][2] * v[2];
       const F ti = zi[i][0] * v[0] + zi[i][1] * v[1] + zi[i][2] * v[2];

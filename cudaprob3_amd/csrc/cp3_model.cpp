@@ -121,15 +121,17 @@ inline Cnt<double> m_fabs(Cnt<double> x) { return mk<double>(std::fabs(x.v)); }
 inline Cnt<float> m_fabs(Cnt<float> x) { return mk<float>(std::fabs(x.v)); }
 inline Cnt<double> m_rint(Cnt<double> x) { g_ops.rint64 += 1; return mk<double>(std::rint(x.v)); }
 inline Cnt<float> m_rint(Cnt<float> x) { return mk<float>(std::rint(x.v)); }
+// The layer phase polynomial e^{2 pi i f} - 1 is counted as one sincos call.
 template <class T>
-inline int m_toint(Cnt<T> x) { return (int)(int64_t)x.v; }
-// The FP32 quarter-revolution sincos is counted as one sincos call.
-inline void m_sincos_quarter(Cnt<float> x, int n, Cnt<float>* s, Cnt<float>* c) {
-  g_ops.sincos32 += 1;
-  float sv, cv;
-  cp3::m_sincos_quarter(x.v, n, &sv, &cv);
-  s->v = sv;
-  c->v = cv;
+inline void m_cispi_m1(Cnt<T> f, Cnt<T>* wr, Cnt<T>* wi) {
+  if (sizeof(T) == 8)
+    g_ops.sincos64 += 1;
+  else
+    g_ops.sincos32 += 1;
+  T r, i;
+  cp3::m_cispi_m1(f.v, &r, &i);
+  wr->v = r;
+  wi->v = i;
 }
 inline void m_sincos(Cnt<double> x, Cnt<double>* s, Cnt<double>* c) {
   g_ops.sincos64 += 1;
@@ -158,13 +160,6 @@ struct CountMixed {
   using F = Cnt<float>;
 };
 
-// The phase rules of cp3_math.hpp with counted scalars.
-template <>
-struct Phase<CountF64> : PhaseRad<Cnt<double>> {};
-template <>
-struct Phase<CountF32> : PhaseQuarter<Cnt<float>, Cnt<float>> {};
-template <>
-struct Phase<CountMixed> : PhaseQuarter<Cnt<double>, Cnt<float>> {};
 }  // namespace cp3
 
 namespace {
@@ -220,6 +215,22 @@ void cp3m_stage_ops(int precision, double* out) {
     stage_ops<cp3::CountF32>(out);
   else
     stage_ops<cp3::CountMixed>(out);
+}
+
+// The layer phase function w = e^{2 pi i f} - 1 (m_cispi_m1) at n points,
+// in FP64 (precision 0) or FP32 (otherwise).
+void cp3m_cispi_m1(int precision, const double* f, int n, double* wr,
+                   double* wi) {
+  for (int k = 0; k < n; ++k) {
+    if (precision == 0) {
+      cp3::m_cispi_m1(f[k], &wr[k], &wi[k]);
+    } else {
+      float r, i;
+      cp3::m_cispi_m1((float)f[k], &r, &i);
+      wr[k] = r;
+      wi[k] = i;
+    }
+  }
 }
 
 // Shell counts per cosine (the M of the trajectory table) for a model.

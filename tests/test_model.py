@@ -106,7 +106,11 @@ def test_stage_counts():
     f64 = FL.stage_ops("fp64")
     mixed = FL.stage_ops("mixed")
     # a middle layer: two rank-one updates of two complex 3-vectors each
-    assert f64["middle"]["f64"] == mixed["middle"]["f32"] + mixed["middle"]["f64"] - 2
+    # (6 + 4 + 6 multiply-adds = 28 FLOPs per update) in F, and per phase
+    # q = om L, q - rint(q) and the determinant sum in D
+    assert mixed["middle"]["f32"] == 4 * 28
+    assert mixed["middle"]["f64"] == 2 * 3
+    assert f64["middle"]["f64"] == mixed["middle"]["f32"] + mixed["middle"]["f64"]
     assert f64["middle"]["sincos64"] == 2 and mixed["middle"]["sincos32"] == 2
     assert f64["solve"]["acos64"] == 1 and mixed["solve"]["acos64"] == 1
     assert mixed["solve"]["f32"] == 0  # the eigen-solve stays FP64
@@ -116,3 +120,28 @@ def test_stage_counts():
     c8 = FL.batch_counts(wl, "mixed", 8)
     assert c8["acos64"] < c1["acos64"] / 6
     assert c8["f32"] == c1["f32"]
+
+
+def test_phase_polynomial():
+    """m_cispi_m1(f) = e^{2 pi i f} - 1 on |f| <= 1/2 against numpy's sin:
+    a few ulp of |w| <= 2 everywhere, exact zero at f = 0, w = -2 at the half
+    revolution, and relative accuracy for tiny f (no cancellation)."""
+    from cudaprob3_amd._native import MODEL_SO
+    m = C.CDLL(MODEL_SO)
+    m.cp3m_cispi_m1.argtypes = [C.c_int, dp, C.c_int, dp, dp]
+    grid = np.linspace(-0.5, 0.5, 100001)
+    tiny = np.array([1e-15, -1e-9, 3e-7, -2e-12])  # squares stay normal
+    f0 = np.concatenate([grid, [0.0], tiny])
+    for prec, tol in ((0, 2e-15), (1, 8e-7)):
+        f = f0 if prec == 0 else f0.astype(np.float32).astype(np.float64)
+        wr, wi = np.zeros_like(f), np.zeros_like(f)
+        m.cp3m_cispi_m1(prec, P(f), len(f), wr.ctypes.data_as(dp),
+                        wi.ctypes.data_as(dp))
+        assert np.abs(wr + 2 * np.sin(np.pi * f) ** 2).max() < tol
+        assert np.abs(wi - np.sin(2 * np.pi * f)).max() < tol
+        n = len(grid)
+        assert wr[n] == 0.0 and wi[n] == 0.0
+        assert abs(wr[0] + 2) < tol and abs(wi[0]) < tol
+        t = f[n + 1:]
+        assert np.all(np.abs(wi[n + 1:] / (2 * np.pi * t) - 1) < 1e-6)
+        assert np.all(np.abs(wr[n + 1:] / (-2 * (np.pi * t) ** 2) - 1) < 1e-5)
