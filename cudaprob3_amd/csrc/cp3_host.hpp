@@ -125,6 +125,21 @@ inline SetConst make_set(double t12, double t13, double t23, double dcp,
     for (int k = 0; k < 3; ++k) acc += O[ii[n]][k] * d[k] * O[jj[n]][k];
     s.hv[n] = acc;
   }
+  // roots in vacuum order: root r = amp (A_r cos t0 + B_r sin t0) + shift
+  const double A[3] = {1.0, -0.5, -0.5};
+  const double B[3] = {0.0, kSqrt3Half, -kSqrt3Half};
+  for (int k = 0; k < 3; ++k) {
+    s.ra[k] = A[s.order[k]];
+    s.rb[k] = B[s.order[k]];
+  }
+  // H'^2 without the matter term's contributions
+  const double* h = s.hv;
+  s.hh[0] = h[1] * h[1] + h[2] * h[2];
+  s.hh[1] = h[1] * h[3] + h[2] * h[4];
+  s.hh[2] = h[1] * h[4] + h[2] * h[5];
+  s.hh[3] = h[1] * h[1] + h[3] * h[3] + h[4] * h[4];
+  s.hh[4] = h[1] * h[2] + h[3] * h[4] + h[4] * h[5];
+  s.hh[5] = h[2] * h[2] + h[4] * h[4] + h[5] * h[5];
   s.sgn = nutype ? 1.0 : -1.0;
   s.c23 = std::cos(t23);
   s.s23 = std::sin(t23);

@@ -227,6 +227,30 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
   }
 }
 
+// Minimum waves per SIMD asked of each atmospheric variant, i.e. a cap of
+// 512 / waves VGPRs per lane (0: the compiler's own allocation).  Chosen from
+// sweeps of every (precision, cosine block, group, grid) on the 1024-set,
+// 59-shell batch on MI355X: a third wave hides the LDS, scalar-load and
+// barrier latency of the mixed NC = 1 and 4 variants (NC = 4, group 4:
+// 12.5 -> 10.6 ms), two waves instead of one help FP64 NC = 4; elsewhere
+// the cap costs more in spills than it gains.
+template <class P, int NC>
+struct MinWaves {
+  static constexpr int value = 0;
+};
+template <int NC>
+struct MinWaves<PolicyMixed, NC> {
+  static constexpr int value = (NC == 1 || NC == 4) ? 3 : 0;
+};
+template <int NC>
+struct MinWaves<PolicyF64, NC> {
+  static constexpr int value = NC == 1 ? 3 : (NC == 4 ? 2 : 0);
+};
+template <int NC>
+struct MinWaves<PolicyF32, NC> {
+  static constexpr int value = NC == 8 ? 3 : 0;
+};
+
 template <class P, int W>
 struct StageOf {
   using type = SlabStage<P, W>;
@@ -237,7 +261,8 @@ struct StageOf<P, 1> {
 };
 
 template <class P, int NC, int W>
-__global__ void __launch_bounds__(kWave * W) atmos_single_kernel(AtmosJob job) {
+__global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value))
+atmos_single_kernel(AtmosJob job) {
   __shared__ typename StageOf<P, W>::type st;
   atmos_unit<P, NC, W>(job, (int64_t)blockIdx.x, st);
 }
@@ -259,7 +284,7 @@ __device__ __forceinline__ int find_job_atmos(const AtmosJob* jobs, int njobs,
 // Batched form over the flattened work list of every job: one workgroup per
 // work unit, or a persistent grid walking the list with a static stride.
 template <class P, int NC, int W>
-__global__ void __launch_bounds__(kWave * W) atmos_batch_kernel(
+__global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_batch_kernel(
     const AtmosJob* __restrict__ jobs, int njobs, int64_t nunits) {
   __shared__ typename StageOf<P, W>::type st;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {

@@ -71,7 +71,15 @@ struct SetConst {
   double sgn;         // sign of the matter term: -1 neutrino, +1 antineutrino
   double c23, s23;    // W = R23 diag(1, 1, e^{i d})
   double cd, sd;      // cos d, sin d (d already negated for kNuType < 0)
-  int32_t order[3];   // vacuum ordering (physics.hpp:232-246)
+  // The cubic's roots in the reference's vacuum order (physics.hpp:232-246,
+  // 327-330): eigenvalue k is amp (ra[k] cos t0 + rb[k] sin t0) + shift, with
+  // (ra, rb) = (1, 0), (-1/2, sqrt(3)/2) or (-1/2, -sqrt(3)/2) for roots 0-2.
+  double ra[3], rb[3];
+  // The parts of H'^2 that do not depend on the matter term (H' differs from
+  // O diag(DM(k,0)) O^T only in H'_00): hh00 + H'_00^2, hh01 + H'_00 hv[1],
+  // hh02 + H'_00 hv[2], and hh11, hh12, hh22 as they are.
+  double hh[6];
+  int32_t order[3];   // vacuum ordering (physics.hpp:232-246), host side
   int32_t pad;
 };
 
@@ -100,6 +108,13 @@ CP3_HD double m_fabs(double x) { return fabs(x); }
 CP3_HD float m_fabs(float x) { return fabsf(x); }
 CP3_HD double m_rint(double x) { return rint(x); }
 CP3_HD float m_rint(float x) { return rintf(x); }
+#if defined(__HIP_DEVICE_COMPILE__)
+CP3_HD double m_rsqrt(double x) { return rsqrt(x); }
+CP3_HD float m_rsqrt(float x) { return rsqrtf(x); }
+#else
+CP3_HD double m_rsqrt(double x) { return 1.0 / sqrt(x); }
+CP3_HD float m_rsqrt(float x) { return 1.0f / sqrtf(x); }
+#endif
 // sin and cos of an FP64 angle, inline (no call, so nothing is spilled
 // around it): Cody-Waite reduction by pi/2 split in three parts (the first
 // two with 33 significant bits, so k*P1 and k*P2 are exact for |k| < 2^20),
@@ -244,76 +259,58 @@ struct Slab {
   typename P::D oma, omb;
 };
 
-template <class D>
-CP3_HD D sel3(int i, D a0, D a1, D a2) {
-  return i == 0 ? a0 : (i == 1 ? a1 : a2);
-}
-
-// Unit eigenvector of the real symmetric matrix H (upper triangle) for the
-// eigenvalue lk, from the Sylvester numerator (H - lp)(H - lq), whose columns
-// are all multiples of v_k (physics.hpp:394-427 builds the same product).
+// Unit eigenvector of the real symmetric matrix H for the eigenvalue lk from
+// the Sylvester numerator P = (H - lp)(H - lq) = H^2 - (lp + lq) H + lp lq,
+// whose columns are all multiples of v_k (physics.hpp:394-427 builds the same
+// product): the column j with the largest P_jj dk = dk^2 v_j^2,
+// dk = (lk - lp)(lk - lq), scaled by 1/sqrt(P_jj dk).  H and H2 = H^2 as
+// upper triangles 00 01 02 11 12 22.
 template <class D, class F>
-CP3_HD void eigvec(const D H[6], D lk, D lp, D lq, F v[3]) {
-  const D a00 = H[0] - lp, a11 = H[3] - lp, a22 = H[5] - lp;
-  const D b00 = H[0] - lq, b11 = H[3] - lq, b22 = H[5] - lq;
-  const D h01 = H[1], h02 = H[2], h12 = H[4];
-  // P = A B with A = H - lp, B = H - lq (both symmetric, commuting).
-  const D p00 = a00 * b00 + h01 * h01 + h02 * h02;
-  const D p11 = h01 * h01 + a11 * b11 + h12 * h12;
-  const D p22 = h02 * h02 + h12 * h12 + a22 * b22;
+CP3_HD void eigvec(const D H[6], const D H2[6], D lk, D lp, D lq, F v[3]) {
+  const D sp = lp + lq, pp = lp * lq;
+  const D p00 = (H2[0] - sp * H[0]) + pp;
+  const D p11 = (H2[3] - sp * H[3]) + pp;
+  const D p22 = (H2[5] - sp * H[5]) + pp;
+  const D p01 = H2[1] - sp * H[1];
+  const D p02 = H2[2] - sp * H[2];
+  const D p12 = H2[4] - sp * H[4];
   const D dk = (lk - lp) * (lk - lq);
-  // P_jj = dk v_j^2: take the column with the largest |v_j|.
   const D s0 = p00 * dk, s1 = p11 * dk, s2 = p22 * dk;
-  D c0, c1, c2, pjj;
-  if (s0 >= s1 && s0 >= s2) {
-    c0 = p00;
-    c1 = h01 * b00 + a11 * h01 + h12 * h02;
-    c2 = h02 * b00 + h12 * h01 + a22 * h02;
-    pjj = s0;
-  } else if (s1 >= s2) {
-    c0 = a00 * h01 + h01 * b11 + h02 * h12;
-    c1 = p11;
-    c2 = h02 * h01 + h12 * b11 + a22 * h12;
-    pjj = s1;
-  } else {
-    c0 = a00 * h02 + h01 * h12 + h02 * b22;
-    c1 = h01 * h02 + a11 * h12 + h12 * b22;
-    c2 = p22;
-    pjj = s2;
-  }
-  const D inv = D(1) / m_sqrt(pjj);
+  const bool j0 = s0 >= s1 && s0 >= s2;
+  const bool j1 = !j0 && s1 >= s2;
+  const D c0 = j0 ? p00 : (j1 ? p01 : p02);
+  const D c1 = j0 ? p01 : (j1 ? p11 : p12);
+  const D c2 = j0 ? p02 : (j1 ? p12 : p22);
+  const D inv = m_rsqrt(j0 ? s0 : (j1 ? s1 : s2));
   v[0] = (F)(c0 * inv);
   v[1] = (F)(c1 * inv);
   v[2] = (F)(c2 * inv);
 }
 
 // Matter eigenvalues in the reference's order: the trigonometric roots of
-// the cubic with physics.hpp:283-323's coefficients, reordered by ORDER.
+// the cubic with physics.hpp:283-323's coefficients, in ORDER's sequence
+// through the per-set root coefficients ra, rb.
 template <class D>
-CP3_HD void matter_eigs(const SetConst& sc, D E, D rhoYe, D lam[3]) {
-  const D fac = D(sc.sgn) * ((D(kGf2) * E) * rhoYe);
+CP3_HD void matter_eigs(const SetConst& sc, D fac, D lam[3]) {
   const D d01 = D(sc.dm01), d02 = D(sc.dm02);
   const D alpha = (fac + d01) + d02;
   const D beta = d01 * d02 + fac * (d01 * D(sc.q1) + d02 * D(sc.q2));
   const D gamma = ((fac * d01) * d02) * D(sc.g0);
   D tmp = alpha * alpha - D(3) * beta;
   tmp = tmp < D(0) ? D(0) : tmp;
+  const D st = m_sqrt(tmp);
   D arg = (D(2) * alpha * alpha * alpha - D(9) * alpha * beta +
            D(27) * gamma) /
-          (D(2) * m_sqrt(tmp * tmp * tmp));
+          (D(2) * (tmp * st));
   if (m_fabs(arg) > D(1)) arg = arg > D(0) ? D(1) : D(-1);
-  const D th0 = m_acos(arg) / D(3);
+  const D th0 = m_acos(arg) * D(1.0 / 3.0);
   D sn, cs;
   m_sincos(th0, &sn, &cs);
-  const D amp = D(-2.0 / 3.0) * m_sqrt(tmp);
-  const D shift = D(sc.dm00) - alpha / D(3);
-  const D hs = D(-0.5) * cs, ks = D(kSqrt3Half) * sn;
-  const D r0 = amp * cs + shift;
-  const D r1 = amp * (hs + ks) + shift;
-  const D r2 = amp * (hs - ks) + shift;
-  lam[0] = sel3(sc.order[0], r0, r1, r2);
-  lam[1] = sel3(sc.order[1], r0, r1, r2);
-  lam[2] = sel3(sc.order[2], r0, r1, r2);
+  const D amp = D(-2.0 / 3.0) * st;
+  const D shift = D(sc.dm00) - alpha * D(1.0 / 3.0);
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    lam[k] = amp * (D(sc.ra[k]) * cs + D(sc.rb[k]) * sn) + shift;
 }
 
 // Solves one slab of electron density rhoYe at energy E (invE = 1/E).
@@ -321,26 +318,27 @@ template <class P>
 CP3_HD void solve_slab(const SetConst& sc, typename P::D E,
                        typename P::D invE, typename P::D rhoYe, Slab<P>& s) {
   using D = typename P::D;
-  D lam[3];
-  matter_eigs<D>(sc, E, rhoYe, lam);
   const D fac = D(sc.sgn) * ((D(kGf2) * E) * rhoYe);
-  const D H[6] = {D(sc.hv[0]) - fac, D(sc.hv[1]), D(sc.hv[2]),
-                  D(sc.hv[3]),       D(sc.hv[4]), D(sc.hv[5])};
+  D lam[3];
+  matter_eigs<D>(sc, fac, lam);
+  const D h0 = D(sc.hv[0]) - fac;
+  const D H[6] = {h0, D(sc.hv[1]), D(sc.hv[2]), D(sc.hv[3]), D(sc.hv[4]),
+                  D(sc.hv[5])};
+  const D H2[6] = {h0 * h0 + D(sc.hh[0]), h0 * D(sc.hv[1]) + D(sc.hh[1]),
+                   h0 * D(sc.hv[2]) + D(sc.hh[2]), D(sc.hh[3]), D(sc.hh[4]),
+                   D(sc.hh[5])};
   // Reference index: one end of the closest pair; `near` is the other end,
   // `far` the remaining eigenvalue.
   const D g01 = m_fabs(lam[0] - lam[1]);
   const D g12 = m_fabs(lam[1] - lam[2]);
   const D g02 = m_fabs(lam[0] - lam[2]);
-  D lr, ln, lf;
-  if (g01 <= g12 && g01 <= g02) {
-    lr = lam[0]; ln = lam[1]; lf = lam[2];
-  } else if (g12 <= g02) {
-    lr = lam[2]; ln = lam[1]; lf = lam[0];
-  } else {
-    lr = lam[2]; ln = lam[0]; lf = lam[1];
-  }
-  eigvec<D>(H, lf, lr, ln, s.va);
-  eigvec<D>(H, ln, lr, lf, s.vb);
+  const bool p01 = g01 <= g12 && g01 <= g02;
+  const bool p12 = !p01 && g12 <= g02;
+  const D lr = p01 ? lam[0] : lam[2];
+  const D ln = (p01 || p12) ? lam[1] : lam[0];
+  const D lf = p01 ? lam[2] : (p12 ? lam[0] : lam[1]);
+  eigvec<D>(H, H2, lf, lr, ln, s.va);
+  eigvec<D>(H, H2, ln, lr, lf, s.vb);
   const D k = D(-kLoE * Phase<P>::kOmScale) * invE;
   s.oma = k * (lf - lr);
   s.omb = k * (ln - lr);

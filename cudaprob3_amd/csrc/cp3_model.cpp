@@ -115,6 +115,8 @@ template <class T> bool operator>=(Cnt<T> a, Cnt<T> b) { return a.v >= b.v; }
 // of cp3_math.hpp.
 inline Cnt<double> m_sqrt(Cnt<double> x) { g_ops.sqrt64 += 1; g_ops.f64 += 1; return mk<double>(std::sqrt(x.v)); }
 inline Cnt<float> m_sqrt(Cnt<float> x) { g_ops.sqrt32 += 1; g_ops.f32 += 1; return mk<float>(std::sqrt(x.v)); }
+inline Cnt<double> m_rsqrt(Cnt<double> x) { g_ops.sqrt64 += 1; g_ops.f64 += 2; return mk<double>(1.0 / std::sqrt(x.v)); }
+inline Cnt<float> m_rsqrt(Cnt<float> x) { g_ops.sqrt32 += 1; g_ops.f32 += 2; return mk<float>(1.0f / std::sqrt(x.v)); }
 inline Cnt<double> m_acos(Cnt<double> x) { g_ops.acos64 += 1; return mk<double>(std::acos(x.v)); }
 inline Cnt<float> m_acos(Cnt<float> x) { g_ops.acos32 += 1; return mk<float>(std::acos(x.v)); }
 inline Cnt<double> m_fabs(Cnt<double> x) { return mk<double>(std::fabs(x.v)); }
