@@ -48,9 +48,14 @@ struct cp3_batch {
   std::vector<cp3::AtmosJob> ajobs;
   std::vector<cp3::BeamJob> bjobs;
   DevBuf d_ajobs, d_bjobs, d_out;
+  // packed tail tiles (cp3::TailGroup) and their job index lists
+  std::vector<cp3::TailGroup> tgroups;
+  std::vector<int32_t> tjobs;
+  std::vector<size_t> tjob_off;
+  DevBuf d_tgroups, d_tjobs;
   PinnedBuf staging;
   std::vector<size_t> out_off, out_len;
-  int64_t a_units = 0, b_units = 0, evals = 0;
+  int64_t a_units = 0, t_units = 0, b_units = 0, evals = 0;
   size_t out_total = 0;
 
   int tune_nc = 0, tune_w = 0, tune_waves = 0;  // 0 = automatic
@@ -108,23 +113,65 @@ struct cp3_batch {
     }
     const size_t abytes = na * sizeof(cp3::AtmosJob);
     const size_t bbytes = nb * sizeof(cp3::BeamJob);
+    const size_t gbytes = params_only ? 0 : tgroups.size() * sizeof(cp3::TailGroup);
+    const size_t jbytes = params_only ? 0 : tjobs.size() * sizeof(int32_t);
     DeviceScope ds(dev);
     // staging may still feed an earlier copy on this stream
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    staging.alloc(abytes + bbytes);
-    if (abytes) std::memcpy(staging.p, ajobs.data(), abytes);
-    if (bbytes) std::memcpy((char*)staging.p + abytes, bjobs.data(), bbytes);
+    staging.alloc(abytes + bbytes + gbytes + jbytes);
+    char* const st = (char*)staging.p;
     if (!params_only) {
       d_ajobs.alloc(abytes, dev);
       d_bjobs.alloc(bbytes, dev);
+      d_tgroups.alloc(gbytes, dev);
+      d_tjobs.alloc(jbytes, dev);
+      for (size_t t = 0; t < tgroups.size(); ++t)
+        tgroups[t].jobs = (const int32_t*)d_tjobs.p + tjob_off[t];
     }
-    if (abytes)
-      hip_check(hipMemcpyAsync(d_ajobs.p, staging.p, abytes, hipMemcpyHostToDevice, stream),
-                "job table upload");
-    if (bbytes)
-      hip_check(hipMemcpyAsync(d_bjobs.p, (char*)staging.p + abytes, bbytes,
-                               hipMemcpyHostToDevice, stream),
-                "job table upload");
+    if (abytes) std::memcpy(st, ajobs.data(), abytes);
+    if (bbytes) std::memcpy(st + abytes, bjobs.data(), bbytes);
+    if (gbytes) std::memcpy(st + abytes + bbytes, tgroups.data(), gbytes);
+    if (jbytes) std::memcpy(st + abytes + bbytes + gbytes, tjobs.data(), jbytes);
+    const struct {
+      void* dst;
+      size_t off, n;
+    } ups[4] = {{d_ajobs.p, 0, abytes},
+                {d_bjobs.p, abytes, bbytes},
+                {d_tgroups.p, abytes + bbytes, gbytes},
+                {d_tjobs.p, abytes + bbytes + gbytes, jbytes}};
+    for (const auto& u : ups)
+      if (u.n)
+        hip_check(hipMemcpyAsync(u.dst, st + u.off, u.n, hipMemcpyHostToDevice, stream),
+                  "job table upload");
+  }
+
+  // Atmospheric jobs sharing a grid whose nE % 64 leaves room for two or
+  // more tails in a wave run their last energies in packed tail tiles; every
+  // other job covers all its energy tiles itself.
+  void plan_tails() {
+    tgroups.clear();
+    tjobs.clear();
+    tjob_off.clear();
+    std::vector<std::vector<int32_t>> by_grid(agrids.size());
+    for (size_t j = 0; j < jobs.size(); ++j)
+      if (jobs[j].kind == CP3_JOB_ATMOS) by_grid[(size_t)jobs[j].grid].push_back(slot[j]);
+    for (size_t gi = 0; gi < agrids.size(); ++gi) {
+      const cp3::GridDev g = agrids[gi]->dev();
+      const int tail = g.nE % cp3::kWave;
+      const int per = tail ? cp3::kWave / tail : 0;
+      const bool pack = per >= 2 && by_grid[gi].size() >= 2;
+      for (int32_t a : by_grid[gi])
+        ajobs[(size_t)a].etiles = pack ? g.nE / cp3::kWave : cp3::energy_tiles(g);
+      if (!pack) continue;
+      cp3::TailGroup t{};
+      t.g = g;
+      t.njobs = (int32_t)by_grid[gi].size();
+      t.tail = tail;
+      t.per = per;
+      tgroups.push_back(t);
+      tjob_off.push_back(tjobs.size());
+      tjobs.insert(tjobs.end(), by_grid[gi].begin(), by_grid[gi].end());
+    }
   }
 
   void check_job(const cp3_job& j) const {
@@ -172,6 +219,7 @@ struct cp3_batch {
     }
     out_total = off;
     d_out.alloc(out_total, dev);
+    plan_tails();
     choose_tuning();
     build_tables(false);
     drop_graph();
@@ -198,9 +246,15 @@ struct cp3_batch {
       if (jobs[j].kind != CP3_JOB_ATMOS) continue;
       cp3::AtmosJob& a = ajobs[(size_t)slot[j]];
       a.unit_begin = u;
-      u += cp3::atmos_units(agrids[(size_t)jobs[j].grid]->dev(), nc, w);
+      u += cp3::cosine_blocks(agrids[(size_t)jobs[j].grid]->dev(), nc, w) * a.etiles;
     }
     a_units = u;
+    u = 0;
+    for (cp3::TailGroup& t : tgroups) {
+      t.unit_begin = u;
+      u += cp3::tail_units(t, nc, w);
+    }
+    t_units = u;
     u = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
       if (jobs[j].kind != CP3_JOB_BEAM) continue;
@@ -244,6 +298,11 @@ struct cp3_batch {
       hip_check(cp3::launch_atmos_batch(precision, nc, w, (const cp3::AtmosJob*)d_ajobs.p,
                                         (int)ajobs.size(), a_units, a_grid, stream),
                 "batched atmospheric launch");
+    if (t_units > 0)
+      hip_check(cp3::launch_atmos_tail(precision, nc, w, (const cp3::AtmosJob*)d_ajobs.p,
+                                       (const cp3::TailGroup*)d_tgroups.p,
+                                       (int)tgroups.size(), t_units, stream),
+                "packed tail launch");
     if (b_units > 0)
       hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)d_bjobs.p,
                                        (int)bjobs.size(), b_units, b_grid, stream),

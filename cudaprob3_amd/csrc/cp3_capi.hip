@@ -149,6 +149,7 @@ struct cp3_atmos {
     job.g = grid();
     job.out = d_out.p;
     job.unit_begin = 0;
+    job.etiles = cp3::energy_tiles(job.g);
     int nc = 1, w = 1;
     auto_tuning(precision, job.g, &nc, &w);
     if (nc_force) nc = nc_force;

@@ -119,20 +119,19 @@ __device__ __forceinline__ void stage_chunk(SlabStage<P, W>& st, int buf, int c,
   }
 }
 
+// The trajectories of cosine block sb of grid g for one energy per lane:
+// lane energy index e (stored only if e_ok), the lane's parameter set sc and
+// output array outp ([9][nC][nE]).  sc and outp are wave-uniform for a job's
+// own work units and per lane in packed tail tiles.
 template <class P, int NC, int W, class Stage>
-__device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
-                                           Stage& st) {
+__device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, bool e_ok,
+                                            const SetConst& sc, void* outp,
+                                            Stage& st) {
   using D = typename P::D;
   using F = typename P::F;
   using O = typename OutT<P>::type;
-  const GridDev& g = job.g;
   const int lane = (int)threadIdx.x % kWave;
   const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-  const int ntile = (g.nE + kWave - 1) / kWave;
-  const int sb = (int)(local / ntile);
-  const int et = (int)(local - (int64_t)sb * ntile);
-  const int e = et * kWave + lane;
-  const bool e_ok = e < g.nE;
   const D E = (D)g.ev[e_ok ? e : g.nE - 1];
   const D inv = D(1) / E;
 
@@ -163,12 +162,12 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
     int i = 0;
     for (; i < Mmin - 1; ++i) {
       Slab<P> s;
-      solve_slab<P>(job.sc, E, inv, (D)uload(g.rye, i), s);
+      solve_slab<P>(sc, E, inv, (D)uload(g.rye, i), s);
       middle_all<P, NC>(t, s, g, cidx, i);
     }
     for (; i < Mmax; ++i) {
       Slab<P> s;
-      solve_slab<P>(job.sc, E, inv, (D)uload(g.rye, i), s);
+      solve_slab<P>(sc, E, inv, (D)uload(g.rye, i), s);
       apply_shell<P, NC>(t, s, g, cidx, Mc, i);
     }
   } else {
@@ -182,7 +181,7 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
     int buf = 0;
     int c = 0;
     for (; c < nfast; ++c) {
-      stage_chunk<P, W>(st, buf, c, wave, Mwg, job.sc, E, inv, g, lane);
+      stage_chunk<P, W>(st, buf, c, wave, Mwg, sc, E, inv, g, lane);
       __syncthreads();
 #pragma unroll
       for (int w = 0; w < W; ++w) {
@@ -193,7 +192,7 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
       buf ^= 1;
     }
     for (; c < nchunk; ++c) {
-      stage_chunk<P, W>(st, buf, c, wave, Mwg, job.sc, E, inv, g, lane);
+      stage_chunk<P, W>(st, buf, c, wave, Mwg, sc, E, inv, g, lane);
       __syncthreads();
 #pragma unroll
       for (int w = 0; w < W; ++w) {
@@ -212,19 +211,48 @@ __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
   }
 
   Slab<P> vac;
-  solve_slab<P>(job.sc, E, inv, D(0), vac);
-  O* out = (O*)job.out;
+  solve_slab<P>(sc, E, inv, D(0), vac);
+  O* out = (O*)outp;
   const int64_t plane = (int64_t)g.nC * g.nE;
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     F prob[9];
-    traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cidx[j]), job.sc, prob);
+    traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cidx[j]), sc, prob);
     if (e_ok && c_ok[j]) {
       O* dst = out + (int64_t)cidx[j] * g.nE + e;
 #pragma unroll
       for (int ch = 0; ch < 9; ++ch) dst[ch * plane] = (O)prob[ch];
     }
   }
+}
+
+// A work unit of one job: cosine block x one of its own 64-energy tiles.
+template <class P, int NC, int W, class Stage>
+__device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
+                                           Stage& st) {
+  const int lane = (int)threadIdx.x % kWave;
+  const int sb = (int)(local / job.etiles);
+  const int et = (int)(local - (int64_t)sb * job.etiles);
+  const int e = et * kWave + lane;
+  atmos_block<P, NC, W>(job.g, sb, e, e < job.g.nE, job.sc, job.out, st);
+}
+
+// A work unit of a tail group: cosine block x packed tile t, whose lanes
+// hold the last tg.tail energies of tg.per jobs.
+template <class P, int NC, int W, class Stage>
+__device__ __forceinline__ void atmos_tail_unit(const AtmosJob* jobs,
+                                                const TailGroup& tg, int64_t local,
+                                                Stage& st) {
+  const int lane = (int)threadIdx.x % kWave;
+  const int ntt = (tg.njobs + tg.per - 1) / tg.per;
+  const int sb = (int)(local / ntt);
+  const int t = (int)(local - (int64_t)sb * ntt);
+  const int q = lane / tg.tail;
+  const int slot = t * tg.per + q;
+  const bool ok = q < tg.per && slot < tg.njobs;
+  const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
+  const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
+  atmos_block<P, NC, W>(tg.g, sb, e, ok, job.sc, job.out, st);
 }
 
 // Minimum waves per SIMD asked of each atmospheric variant, i.e. a cap of
@@ -291,6 +319,26 @@ __global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_bat
     const int j = find_job_atmos(jobs, njobs, u);
     const AtmosJob& job = jobs[j];
     atmos_unit<P, NC, W>(job, u - job.unit_begin, st);
+  }
+}
+
+// The packed tail tiles of a batch (TailGroup), one workgroup per unit.
+template <class P, int NC, int W>
+__global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_tail_kernel(
+    const AtmosJob* __restrict__ jobs, const TailGroup* __restrict__ groups, int ngroups,
+    int64_t nunits) {
+  __shared__ typename StageOf<P, W>::type st;
+  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (groups[mid].unit_begin <= u)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const TailGroup& tg = groups[lo];
+    atmos_tail_unit<P, NC, W>(jobs, tg, u - tg.unit_begin, st);
   }
 }
 
@@ -390,6 +438,35 @@ hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* jobs
     case kFP64: return atmos_batch_p<PolicyF64>(nc, w, jobs, njobs, nunits, grid, st);
     case kFP32: return atmos_batch_p<PolicyF32>(nc, w, jobs, njobs, nunits, grid, st);
     case kMixed: return atmos_batch_p<PolicyMixed>(nc, w, jobs, njobs, nunits, grid, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class P>
+static hipError_t atmos_tail_p(int nc, int w, const AtmosJob* jobs,
+                               const TailGroup* groups, int ngroups, int64_t nunits,
+                               hipStream_t st) {
+  if (nunits <= 0 || ngroups <= 0) return hipSuccess;
+  if (nunits > 0x7fffffff) return hipErrorInvalidConfiguration;
+  const dim3 g((unsigned)nunits);
+#define X(NC_, W_)                                                            \
+  if (nc == NC_ && w == W_) {                                                 \
+    atmos_tail_kernel<P, NC_, W_><<<g, dim3(kWave * W_), 0, st>>>(jobs, groups, \
+                                                                  ngroups, nunits); \
+    return hipGetLastError();                                                 \
+  }
+  CP3_NCW_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* jobs,
+                             const TailGroup* groups, int ngroups, int64_t nunits,
+                             hipStream_t st) {
+  switch (precision) {
+    case kFP64: return atmos_tail_p<PolicyF64>(nc, w, jobs, groups, ngroups, nunits, st);
+    case kFP32: return atmos_tail_p<PolicyF32>(nc, w, jobs, groups, ngroups, nunits, st);
+    case kMixed: return atmos_tail_p<PolicyMixed>(nc, w, jobs, groups, ngroups, nunits, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -33,6 +33,23 @@ struct AtmosJob {
   GridDev g;
   void* out;
   int64_t unit_begin;  // first work unit of this job in a batch
+  // 64-energy tiles this job's own work units cover: all of them
+  // (energy_tiles), or only the full ones when the last nE % 64 energies
+  // run in a packed tail tile (TailGroup)
+  int32_t etiles, pad;
+};
+
+// Packed tail tiles: the last `tail` = nE % 64 energies of up to `per` =
+// 64 / tail jobs that share one grid, side by side in the lanes of one wave
+// (lane l: job jobs[t * per + l / tail], energy nE - tail + l % tail), so the
+// partial last tile of each job does not leave 64 - tail lanes idle.
+// Cosine-dependent data stay wave-uniform; only the parameter set and the
+// output array differ between lanes.
+struct TailGroup {
+  GridDev g;
+  const int32_t* jobs;  // [njobs] indices into the AtmosJob table
+  int32_t njobs, tail, per, pad;
+  int64_t unit_begin;   // first work unit of this group in the tail launch
 };
 
 // One beam evaluation: a parameter set and neutrino type on an energy list
@@ -49,11 +66,19 @@ struct BeamJob {
 
 constexpr int kWave = 64;
 
-// Work units of one job: (w cosine blocks of nc) x (64-energy tile).
+// Cosine blocks of one grid (w waves x nc cosines each) and 64-energy tiles.
+inline int64_t cosine_blocks(const GridDev& g, int nc, int w) {
+  return (g.nC + nc * w - 1) / (nc * w);
+}
+inline int32_t energy_tiles(const GridDev& g) { return (g.nE + kWave - 1) / kWave; }
+
+// Work units of one job over all its energy tiles: (cosine block) x (tile).
 inline int64_t atmos_units(const GridDev& g, int nc, int w = 1) {
-  const int64_t cb = (g.nC + nc * w - 1) / (nc * w);
-  const int64_t et = (g.nE + kWave - 1) / kWave;
-  return cb * et;
+  return cosine_blocks(g, nc, w) * energy_tiles(g);
+}
+// Work units of a tail group: (cosine block) x (packed tile).
+inline int64_t tail_units(const TailGroup& t, int nc, int w) {
+  return cosine_blocks(t.g, nc, w) * ((t.njobs + t.per - 1) / t.per);
 }
 inline int64_t beam_units(int nE) { return (nE + kWave - 1) / kWave; }
 
@@ -66,6 +91,10 @@ hipError_t launch_atmos_single(int precision, int nc, int w, const AtmosJob& job
 hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* d_jobs,
                               int njobs, int64_t nunits, int grid,
                               hipStream_t stream);
+// The packed tail tiles of a batch: one workgroup per unit.
+hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* d_jobs,
+                             const TailGroup* d_groups, int ngroups, int64_t nunits,
+                             hipStream_t stream);
 hipError_t launch_beam_single(int precision, const BeamJob& job,
                               hipStream_t stream);
 hipError_t launch_beam_batch(int precision, const BeamJob* d_jobs, int njobs,

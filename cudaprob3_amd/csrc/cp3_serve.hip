@@ -213,6 +213,7 @@ struct cp3_server {
         a.g = agrids[(size_t)in.grid]->dev();
         a.out = out;
         a.unit_begin = au;
+        a.etiles = cp3::energy_tiles(a.g);
         au += cp3::atmos_units(a.g, nc, w);
         s.ajobs.push_back(a);
       } else {

@@ -154,3 +154,33 @@ def test_batch_errors():
     with pytest.raises(Cp3Error, match="out of range"):
         b.result_device(5)
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("tuning", [(1, 1, 0), (4, 4, -1), (8, 2, 0)])
+def test_packed_tails_edge_cases(precision, tuning):
+    """Jobs sharing a grid run their last nE % 64 energies in packed tail
+    tiles: a grid with fewer than 64 energies (every lane in tails, several
+    tiles, the last one partial), a tail above 32 (not packed), an exact
+    multiple of 64 (no tail) and a lone job on a grid (not packed); every
+    result equals the single propagator's bitwise."""
+    earth = W.earth_59()
+    sets = W.draw_param_sets(4, 55)
+    grids = {"c": (W.cos_grid(11), W.log_grid(20, 0.5, 50.0)),
+             "d": (W.cos_grid(9), W.log_grid(100, 0.5, 50.0)),
+             "e": (W.cos_grid(5), W.log_grid(64, 0.5, 50.0)),
+             "f": (W.cos_grid(7), W.log_grid(90, 0.5, 50.0))}
+    plan = [("c", s, nt) for s in sets for nt in (0, 1)][:7]
+    plan += [("d", sets[0], 0), ("d", sets[1], 1), ("e", sets[2], 0),
+             ("e", sets[3], 1), ("f", sets[0], 1)]
+    b = Batch(precision)
+    gid = {k: b.add_atmos_grid(c, e, earth, 22.0) for k, (c, e) in grids.items()}
+    jobs = [W.AtmosJob(p, grids[g][0], grids[g][1], nt, grid=g) for g, p, nt in plan]
+    b.set_jobs([make_job(ATMOS, gid[j.grid], j.params, j.nutype) for j in jobs])
+    b.set_tuning(*tuning)
+    b.run()
+    b.wait()
+    for i, job in enumerate(jobs):
+        ref = single_atmos(job, earth, precision)
+        assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
+    b.close()
