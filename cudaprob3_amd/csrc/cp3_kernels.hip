@@ -104,7 +104,8 @@ __device__ __forceinline__ void stage_get(const SlabStage<P, W>& st, int buf,
 struct NoStage {};
 
 // Wave `wave`'s share of chunk c: the solve of shell c * W + wave, if the
-// unit crosses it, published in LDS buffer buf.
+// unit crosses it, published in LDS buffer buf.  Shell Mwg stands for the
+// vacuum (the atmosphere segment), solved once and shared like the others.
 template <class P, int W>
 __device__ __forceinline__ void stage_chunk(SlabStage<P, W>& st, int buf, int c,
                                             int wave, int Mwg, const SetConst& sc,
@@ -112,9 +113,10 @@ __device__ __forceinline__ void stage_chunk(SlabStage<P, W>& st, int buf, int c,
                                             const GridDev& g, int lane) {
   using D = typename P::D;
   const int is = c * W + wave;
-  if (is < Mwg) {
+  if (is <= Mwg) {
+    const D rye = (D)uload(g.rye, is < Mwg ? is : 0);
     Slab<P> s;
-    solve_slab<P>(sc, E, inv, (D)uload(g.rye, is), s);
+    solve_slab<P>(sc, E, inv, is < Mwg ? rye : D(0), s);
     stage_put<P, W>(st, buf, wave, lane, s);
   }
 }
@@ -157,6 +159,7 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
   // Shells below Mmin - 1 run in a loop of their own: joining the
   // branch-free path with the general one inside a single loop costs
   // register copies of the whole trajectory state at every shell.
+  Slab<P> vac;
   if constexpr (W == 1) {
     (void)st;
     int i = 0;
@@ -170,10 +173,12 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
       solve_slab<P>(sc, E, inv, (D)uload(g.rye, i), s);
       apply_shell<P, NC>(t, s, g, cidx, Mc, i);
     }
+    solve_slab<P>(sc, E, inv, D(0), vac);
   } else {
     // deepest trajectory of the unit: its first cosine (decreasing order)
     const int Mwg = uload(g.M, uload(g.perm, base));
-    const int nchunk = (Mwg + W - 1) / W;
+    // Mwg shells and the vacuum
+    const int nchunk = Mwg / W + 1;
     // chunks of W shells that are middle shells of every trajectory of this
     // wave; nfast < nchunk since Mmin <= Mwg.  Every wave passes nchunk
     // barriers whatever its own split.
@@ -205,13 +210,13 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
       }
       buf ^= 1;
     }
+    // the vacuum slab, staged as shell Mwg of chunk Mwg / W
+    stage_get<P, W>(st, (Mwg / W) & 1, Mwg % W, lane, vac);
     // the next unit of a persistent workgroup restarts at buffer 0: every
     // wave must be done reading this unit's last chunk first
     __syncthreads();
   }
 
-  Slab<P> vac;
-  solve_slab<P>(sc, E, inv, D(0), vac);
   O* out = (O*)outp;
   const int64_t plane = (int64_t)g.nC * g.nE;
 #pragma unroll

@@ -412,24 +412,57 @@ CP3_HD void traj_middle(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   rank1(s.vb, wr, wi, t.y1);
 }
 
-// Bilinear (unconjugated) dot products.
-template <class F>
-CP3_HD void dot_cc(const C3<F>& p, const C3<F>& q, F& r, F& i) {
-  r = p.re[0] * q.re[0] - p.im[0] * q.im[0];
-  i = p.re[0] * q.im[0] + p.im[0] * q.re[0];
-  r += p.re[1] * q.re[1] - p.im[1] * q.im[1];
-  i += p.re[1] * q.im[1] + p.im[1] * q.re[1];
-  r += p.re[2] * q.re[2] - p.im[2] * q.im[2];
-  i += p.re[2] * q.im[2] + p.im[2] * q.re[2];
-}
 template <class F>
 CP3_HD void dot_cr(const C3<F>& p, const F v[3], F& r, F& i) {
   r = p.re[0] * v[0] + p.re[1] * v[1] + p.re[2] * v[2];
   i = p.im[0] * v[0] + p.im[1] * v[1] + p.im[2] * v[2];
 }
 
+// z += c u u^T for complex c and u (packed upper triangle 00 01 02 11 12 22),
+// as chains of single products so that every step is one fused multiply-add.
+template <class F>
+CP3_HD void sym_update(F zr[6], F zi[6], const F ur[3], const F ui[3], F cr, F ci) {
+  F sr[3], si[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    sr[i] = cr * ur[i];
+    sr[i] = sr[i] - ci * ui[i];
+    si[i] = cr * ui[i];
+    si[i] = si[i] + ci * ur[i];
+  }
+#pragma unroll
+  for (int n = 0; n < 6; ++n) {
+    const int i = n < 3 ? 0 : (n < 5 ? 1 : 2);
+    const int j = n < 3 ? n : (n < 5 ? n - 2 : 2);
+    zr[n] = zr[n] + sr[i] * ur[j];
+    zr[n] = zr[n] - si[i] * ui[j];
+    zi[n] = zi[n] + sr[i] * ui[j];
+    zi[n] = zi[n] + si[i] * ur[j];
+  }
+}
+
+// Bilinear (unconjugated) dot product p . q of complex 3-vectors, as chains
+// of single products (every step one fused multiply-add).
+template <class F>
+CP3_HD void dot_cc(const C3<F>& p, const C3<F>& q, F& r, F& i) {
+  r = p.re[0] * q.re[0];
+  i = p.re[0] * q.im[0];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    if (m > 0) {
+      r = r + p.re[m] * q.re[m];
+      i = i + p.re[m] * q.im[m];
+    }
+    r = r - p.im[m] * q.im[m];
+    i = i + p.im[m] * q.re[m];
+  }
+}
+
 // The innermost shell, crossed once along its full chord
-// (physics.hpp:979-983, 785): Z = Y^T T_M Y.
+// (physics.hpp:979-983, 785): Z = Y^T T_M Y = Y^T Y + sum_k w_k u_k u_k^T
+// with u_k = Y^T v_k over the two non-reference eigenvectors (an error in an
+// ill-conditioned eigenvector of a near-degenerate pair only enters through
+// its small w_k).
 template <class P>
 CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   using F = typename P::F;
@@ -437,21 +470,24 @@ CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   F er, ei;
   Phase<P>::unit(t.acc, er, ei);
   C3<F> y2;
+  const C3<F>& a = t.y0;
+  const C3<F>& b = t.y1;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     const int p = (m + 1) % 3, q = (m + 2) % 3;
-    const C3<F>& a = t.y0;
-    const C3<F>& b = t.y1;
-    const F xr = a.re[p] * b.re[q] - a.im[p] * b.im[q] -
-                 (a.re[q] * b.re[p] - a.im[q] * b.im[p]);
-    const F xi = a.re[p] * b.im[q] + a.im[p] * b.re[q] -
-                 (a.re[q] * b.im[p] + a.im[q] * b.re[p]);
+    F xr = a.re[p] * b.re[q];
+    xr = xr - a.im[p] * b.im[q];
+    xr = xr - a.re[q] * b.re[p];
+    xr = xr + a.im[q] * b.im[p];
+    F xi = a.re[p] * b.im[q];
+    xi = xi + a.im[p] * b.re[q];
+    xi = xi - a.re[q] * b.im[p];
+    xi = xi - a.im[q] * b.re[p];
     // e * conj(x)
     y2.re[m] = er * xr + ei * xi;
     y2.im[m] = ei * xr - er * xi;
   }
-  // Z = Y^T Y + sum_k w_k u_k u_k^T with u_k = Y^T v_k; packed upper
-  // triangle 00 01 02 11 12 22.
+  // packed upper triangle 00 01 02 11 12 22
   F zr[6], zi[6];
   dot_cc(t.y0, t.y0, zr[0], zi[0]);
   dot_cc(t.y0, t.y1, zr[1], zi[1]);
@@ -462,24 +498,12 @@ CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const F* v = k == 0 ? s.va : s.vb;
-    F wr, wi;
+    F wr, wi, ur[3], ui[3];
     Phase<P>::eval(k == 0 ? s.oma : s.omb, L, wr, wi);
-    F ur[3], ui[3], sr[3], si[3];
     dot_cr(t.y0, v, ur[0], ui[0]);
     dot_cr(t.y1, v, ur[1], ui[1]);
     dot_cr(y2, v, ur[2], ui[2]);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      sr[i] = wr * ur[i] - wi * ui[i];
-      si[i] = wr * ui[i] + wi * ur[i];
-    }
-#pragma unroll
-    for (int n = 0; n < 6; ++n) {
-      const int i = n < 3 ? 0 : (n < 5 ? 1 : 2);
-      const int j = n < 3 ? n : (n < 5 ? n - 2 : 2);
-      zr[n] += sr[i] * ur[j] - si[i] * ui[j];
-      zi[n] += sr[i] * ui[j] + si[i] * ur[j];
-    }
+    sym_update(zr, zi, ur, ui, wr, wi);
   }
   t.y0.re[0] = zr[0]; t.y0.im[0] = zi[0];
   t.y0.re[1] = zr[1]; t.y0.im[1] = zi[1];

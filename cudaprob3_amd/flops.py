@@ -76,11 +76,11 @@ def grid_stage_calls(M: np.ndarray, n_e: int, nc: int, w: int = 1) -> dict:
     """Stage calls of one atmospheric job: cosines ordered by decreasing M
     (the kernel's block order), work units of w waves x nc trajectories,
     every valid energy lane: one solve per shell down to the unit's deepest
-    trajectory plus one atmosphere solve per wave."""
+    trajectory plus one atmosphere (vacuum) solve per unit."""
     Ms = np.sort(M)[::-1]
     per = nc * w
     nunits = (len(Ms) + per - 1) // per
-    solves = sum(int(Ms[u * per]) + w for u in range(nunits))
+    solves = sum(int(Ms[u * per]) + 1 for u in range(nunits))
     return {
         "solve": solves * n_e,
         "middle": int(np.maximum(Ms - 1, 0).sum()) * n_e,
