@@ -35,6 +35,8 @@ PEAK_HBM = 8.0e12    # B/s
 
 
 def build_batch(precision, wl, tuning=None, graph=True):
+    """The batch of every job of wl; tuning None = autotuned, else
+    (cosine block, group waves, workgroups per CU[, solve mode])."""
     b = Batch(precision)
     grids = {}
     for j in wl.jobs:
@@ -43,11 +45,13 @@ def build_batch(precision, wl, tuning=None, graph=True):
     b.set_jobs([make_job(ATMOS, grids[j.grid], j.params, j.nutype)
                 for j in wl.jobs])
     if tuning is None:
-        tuning = b.autotune(reps=2)
+        b.autotune(reps=2)
     else:
-        b.set_tuning(*tuning)
+        if len(tuning) > 3:
+            b.set_solve_mode(tuning[3])
+        b.set_tuning(*[int(x) for x in tuning[:3]])
     b.use_graph(graph)
-    return b, tuning
+    return b, b.tuning()
 
 
 def time_steps(b, steps, warmup):
@@ -143,7 +147,8 @@ def main():
                     help="also run a device-to-device copy of this size (for "
                          "scaling the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--tune", default="auto",
-                    help="'auto' (autotune) or 'NC,GROUP,WGS_PER_CU' for both forms")
+                    help="'auto' (autotune) or 'NC,GROUP,WGS_PER_CU[,MODE]' for "
+                         "both forms (MODE: shared or table)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -160,12 +165,12 @@ def main():
 
     fixed = None
     if args.tune != "auto":
-        fixed = tuple(int(x) for x in args.tune.split(","))
+        fixed = tuple(args.tune.split(","))
     mixed, tune_m = build_batch("mixed", wl, tuning=fixed, graph=graph)
     ms_step, kms = time_steps(mixed, args.steps, args.warmup)
     value = evals / (ms_step / 1e3)
 
-    counts = FL.batch_counts(wl, "mixed", tune_m[0], tune_m[1])
+    counts = FL.batch_counts(wl, "mixed", tune_m[0], tune_m[1], tune_m[3])
     t_k = float(np.mean(kms)) / 1e3
     achieved = (counts["f64"] + counts["f32"] * PEAK_F64 / PEAK_F32) / t_k
     out_bytes = evals * 9 * 4
@@ -193,7 +198,7 @@ def main():
             "jobs": len(wl.jobs), "evals_per_step": evals,
             "precision": "mixed",
             "tuning": {"cosine_block": tune_m[0], "group_waves": tune_m[1],
-                       "workgroups_per_cu": tune_m[2]},
+                       "workgroups_per_cu": tune_m[2], "solve_mode": tune_m[3]},
             "parallelism": "1 GPU",
         },
         "kernel_ms": {"mean": float(np.mean(kms)),
@@ -223,7 +228,7 @@ def main():
         fp64, tune_d = build_batch("fp64", wl, tuning=fixed, graph=graph)
         ms64, kms64 = time_steps(fp64, max(3, args.steps // 2), 1)
         v64 = evals / (ms64 / 1e3)
-        c64 = FL.batch_counts(wl, "fp64", tune_d[0], tune_d[1])
+        c64 = FL.batch_counts(wl, "fp64", tune_d[0], tune_d[1], tune_d[3])
         t64 = float(np.mean(kms64)) / 1e3
         # mixed vs FP64 on the GPU, over sampled jobs
         worst, sq, n = 0.0, 0.0, 0
@@ -237,7 +242,7 @@ def main():
             "value": v64, "ms_per_step": ms64,
             "kernel_ms": float(np.mean(kms64)),
             "tuning": {"cosine_block": tune_d[0], "group_waves": tune_d[1],
-                       "workgroups_per_cu": tune_d[2]},
+                       "workgroups_per_cu": tune_d[2], "solve_mode": tune_d[3]},
             "frac_of_fp64_peak": c64["f64"] / t64 / PEAK_F64,
         }
         result["mixed_over_fp64"] = value / v64

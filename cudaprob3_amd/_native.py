@@ -14,6 +14,9 @@ HIP_SO = os.path.join(PKG, "libcp3hip.so")
 MODEL_SO = os.path.join(PKG, "libcp3model.so")
 
 CP3_FP64, CP3_FP32, CP3_MIXED = 0, 1, 2
+CP3_SOLVE_AUTO, CP3_SOLVE_SHARED, CP3_SOLVE_TABLE = 0, 1, 2
+SOLVE_MODES = {"auto": CP3_SOLVE_AUTO, "shared": CP3_SOLVE_SHARED,
+               "table": CP3_SOLVE_TABLE}
 PRECISIONS = {"fp64": CP3_FP64, "fp32": CP3_FP32, "mixed": CP3_MIXED}
 
 CP3_OK = 0
@@ -100,6 +103,8 @@ _SIGS.update({
     "cp3_batch_update_jobs": (C.c_int, [_vp, C.POINTER(Cp3Job), C.c_int]),
     "cp3_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "cp3_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
+    "cp3_batch_set_solve_mode": (C.c_int, [_vp, C.c_int]),
+    "cp3_batch_get_tuning": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "cp3_batch_use_graph": (C.c_int, [_vp, C.c_int]),
     "cp3_batch_run": (C.c_int, [_vp]),
     "cp3_batch_wait": (C.c_int, [_vp]),

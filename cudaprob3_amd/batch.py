@@ -91,6 +91,22 @@ class Batch:
                                                int(group_waves),
                                                int(workgroups_per_cu)))
 
+    def set_solve_mode(self, mode="auto"):
+        """Where the eigen-solutions come from: "shared" (solved in the
+        propagation kernel, shared through LDS by a work unit), "table"
+        (pre-solved once per job, shell and energy into device memory) or
+        "auto" (the table when it fits)."""
+        N.check(self._lib.cp3_batch_set_solve_mode(self._h,
+                                                   N.SOLVE_MODES[mode]))
+
+    def tuning(self):
+        """(cosine block, group waves, workgroups per CU, solve mode) in
+        effect."""
+        v = [C.c_int() for _ in range(4)]
+        N.check(self._lib.cp3_batch_get_tuning(self._h, *[C.byref(x) for x in v]))
+        names = {m: k for k, m in N.SOLVE_MODES.items()}
+        return v[0].value, v[1].value, v[2].value, names[v[3].value]
+
     def autotune(self, reps=2):
         nc, g, w = C.c_int(), C.c_int(), C.c_int()
         N.check(self._lib.cp3_batch_autotune(self._h, int(reps), C.byref(nc),

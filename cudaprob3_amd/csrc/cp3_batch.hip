@@ -57,9 +57,16 @@ struct cp3_batch {
   std::vector<size_t> out_off, out_len;
   int64_t a_units = 0, t_units = 0, b_units = 0, evals = 0;
   size_t out_total = 0;
+  // slab tables (CP3_SOLVE_TABLE): per atmospheric job its byte offset in
+  // d_slabs, and the pre-pass size
+  std::vector<size_t> slab_off;
+  size_t slab_total = 0;
+  int64_t s_waves = 0;
+  DevBuf d_slabs;
 
   int tune_nc = 0, tune_w = 0, tune_waves = 0;  // 0 = automatic
-  int nc = 1, w = 1, a_grid = 0, b_grid = 0;
+  int tune_mode = CP3_SOLVE_AUTO;
+  int nc = 1, w = 1, mode = CP3_SOLVE_SHARED, a_grid = 0, b_grid = 0;
 
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -86,6 +93,7 @@ struct cp3_batch {
   // `params_only`) and uploads them through pinned staging on the stream.
   void build_tables(bool params_only) {
     const size_t na = ajobs.size(), nb = bjobs.size();
+    if (!params_only && mode == CP3_SOLVE_TABLE) d_slabs.alloc(slab_total, dev);
     for (size_t j = 0; j < jobs.size(); ++j) {
       const cp3_job& in = jobs[j];
       const cp3::SetConst sc =
@@ -97,6 +105,9 @@ struct cp3_batch {
         if (!params_only) {
           a.g = agrids[(size_t)in.grid]->dev();
           a.out = (char*)d_out.p + out_off[j];
+          a.slabs = mode == CP3_SOLVE_TABLE
+                        ? (void*)((char*)d_slabs.p + slab_off[(size_t)slot[j]])
+                        : nullptr;
         }
       } else {
         cp3::BeamJob& b = bjobs[(size_t)slot[j]];
@@ -143,6 +154,34 @@ struct cp3_batch {
       if (u.n)
         hip_check(hipMemcpyAsync(u.dst, st + u.off, u.n, hipMemcpyHostToDevice, stream),
                   "job table upload");
+  }
+
+  // Slab-table layout: job a's table at slab_off[a], [S + 1][tiles * 64]
+  // entries; its pre-pass waves from slab_begin.
+  void plan_slabs() {
+    const size_t entry = cp3::slab_entry_bytes(precision);
+    slab_off.assign(ajobs.size(), 0);
+    size_t off = 0;
+    int64_t wv = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+      if (jobs[j].kind != CP3_JOB_ATMOS) continue;
+      const size_t a = (size_t)slot[j];
+      const cp3::GridDev g = agrids[(size_t)jobs[j].grid]->dev();
+      slab_off[a] = off;
+      ajobs[a].slab_begin = wv;
+      const int64_t waves = cp3::slab_waves(g);
+      off = align_up(off + (size_t)waves * cp3::kWave * entry);
+      wv += waves;
+    }
+    slab_total = off;
+    s_waves = wv;
+  }
+
+  // Table mode unless the tables would take more than kTableMaxBytes.
+  static constexpr size_t kTableMaxBytes = (size_t)16 << 30;
+  int auto_mode() const {
+    return slab_total > 0 && slab_total <= kTableMaxBytes ? CP3_SOLVE_TABLE
+                                                          : CP3_SOLVE_SHARED;
   }
 
   // Atmospheric jobs sharing a grid whose nE % 64 leaves room for two or
@@ -220,6 +259,7 @@ struct cp3_batch {
     out_total = off;
     d_out.alloc(out_total, dev);
     plan_tails();
+    plan_slabs();
     choose_tuning();
     build_tables(false);
     drop_graph();
@@ -239,8 +279,11 @@ struct cp3_batch {
 
   // Unit prefix sums for the chosen cosine block, and grid sizes.
   void choose_tuning() {
-    nc = tune_nc ? tune_nc : auto_nc();
-    w = tune_w ? tune_w : 4;
+    mode = tune_mode ? tune_mode : auto_mode();
+    if (mode == CP3_SOLVE_TABLE && slab_total == 0) mode = CP3_SOLVE_SHARED;
+    nc = tune_nc ? tune_nc : (mode == CP3_SOLVE_TABLE ? 4 : auto_nc());
+    // the table replaces the waves' shared solves: one wave per workgroup
+    w = mode == CP3_SOLVE_TABLE ? 1 : (tune_w ? tune_w : 4);
     int64_t u = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
       if (jobs[j].kind != CP3_JOB_ATMOS) continue;
@@ -265,6 +308,11 @@ struct cp3_batch {
     b_units = u;
     a_grid = grid_for(a_units, tune_waves, true);
     b_grid = grid_for(b_units, tune_waves, false);
+    if (mode == CP3_SOLVE_TABLE && a_grid > 0) {
+      // a multiple of 8 (one share per XCD), at most one block per unit
+      const int64_t cap = (a_units + 7) / 8 * 8;
+      a_grid = (int)std::min<int64_t>(cap, ((int64_t)a_grid + 7) / 8 * 8);
+    }
   }
 
   int auto_nc() const {
@@ -284,7 +332,9 @@ struct cp3_batch {
     if (units <= 0) return 0;
     if (waves < 0 || units <= (int64_t)cus) return (int)std::min<int64_t>(units, 0x7fffffff);
     int occ = 0;
-    if (atmos)
+    if (atmos && mode == CP3_SOLVE_TABLE)
+      hip_check(cp3::atmos_table_occupancy(precision, nc, &occ), "occupancy");
+    else if (atmos)
       hip_check(cp3::atmos_batch_occupancy(precision, nc, w, &occ), "occupancy");
     else
       hip_check(cp3::beam_batch_occupancy(precision, &occ), "occupancy");
@@ -294,6 +344,26 @@ struct cp3_batch {
   }
 
   void enqueue() {
+    if (mode == CP3_SOLVE_TABLE) {
+      const cp3::AtmosJob* aj = (const cp3::AtmosJob*)d_ajobs.p;
+      const int na = (int)ajobs.size();
+      if (s_waves > 0)
+        hip_check(cp3::launch_slab_table(precision, aj, na, s_waves, stream),
+                  "slab table launch");
+      if (a_units > 0)
+        hip_check(cp3::launch_atmos_table(precision, nc, aj, na, a_units, a_grid, stream),
+                  "table-mode atmospheric launch");
+      if (t_units > 0)
+        hip_check(cp3::launch_tail_table(precision, nc, aj,
+                                         (const cp3::TailGroup*)d_tgroups.p,
+                                         (int)tgroups.size(), t_units, stream),
+                  "table-mode tail launch");
+      if (b_units > 0)
+        hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)d_bjobs.p,
+                                         (int)bjobs.size(), b_units, b_grid, stream),
+                  "batched beam launch");
+      return;
+    }
     if (a_units > 0)
       hip_check(cp3::launch_atmos_batch(precision, nc, w, (const cp3::AtmosJob*)d_ajobs.p,
                                         (int)ajobs.size(), a_units, a_grid, stream),
@@ -358,36 +428,50 @@ struct cp3_batch {
     use_graph = false;
     drop_graph();
     float best = 1e30f;
-    int bn = 1, bw = 1, bwv = 0;
+    int bn = 1, bw = 1, bwv = 0, bm = CP3_SOLVE_SHARED;
     const bool any_atmos = !ajobs.empty();
     const int ncs[4] = {1, 2, 4, 8};
     const int gws[3] = {1, 2, 4};
     const int waves[2] = {0, -1};
-    for (int ci = 0; ci < (any_atmos ? 4 : 1); ++ci)
-      for (int gi = 0; gi < (any_atmos ? 3 : 1); ++gi)
-        for (int wi = 0; wi < 2; ++wi) {
-          tune_nc = any_atmos ? ncs[ci] : 0;
-          tune_w = any_atmos ? gws[gi] : 0;
-          tune_waves = waves[wi];
-          choose_tuning();
-          build_tables(false);
-          run();  // warm-up
-          wait();
-          float tot = 0.f;
-          for (int r = 0; r < reps; ++r) {
-            run();
-            tot += last_ms();
+    // the modes tried: a fixed one, or both (table mode only if it fits)
+    int modes[2] = {CP3_SOLVE_SHARED, CP3_SOLVE_TABLE}, nm = 2;
+    if (tune_mode != CP3_SOLVE_AUTO) {
+      modes[0] = tune_mode;
+      nm = 1;
+    } else if (auto_mode() != CP3_SOLVE_TABLE) {
+      nm = 1;
+    }
+    if (!any_atmos) nm = 1;
+    const int fixed_mode = tune_mode;
+    for (int mi = 0; mi < nm; ++mi)
+      for (int ci = 0; ci < (any_atmos ? 4 : 1); ++ci)
+        for (int gi = 0; gi < (any_atmos && modes[mi] == CP3_SOLVE_SHARED ? 3 : 1); ++gi)
+          for (int wi = 0; wi < 2; ++wi) {
+            tune_mode = any_atmos ? modes[mi] : fixed_mode;
+            tune_nc = any_atmos ? ncs[ci] : 0;
+            tune_w = any_atmos && modes[mi] == CP3_SOLVE_SHARED ? gws[gi] : 0;
+            tune_waves = waves[wi];
+            choose_tuning();
+            build_tables(false);
+            run();  // warm-up
+            wait();
+            float tot = 0.f;
+            for (int r = 0; r < reps; ++r) {
+              run();
+              tot += last_ms();
+            }
+            if (tot / reps < best) {
+              best = tot / reps;
+              bn = tune_nc;
+              bw = tune_w;
+              bwv = tune_waves;
+              bm = tune_mode;
+            }
           }
-          if (tot / reps < best) {
-            best = tot / reps;
-            bn = tune_nc;
-            bw = tune_w;
-            bwv = tune_waves;
-          }
-        }
     tune_nc = bn;
     tune_w = bw;
     tune_waves = bwv;
+    tune_mode = fixed_mode == CP3_SOLVE_AUTO ? bm : fixed_mode;
     choose_tuning();
     build_tables(false);
     use_graph = had_graph;
@@ -508,6 +592,31 @@ int cp3_batch_set_tuning(cp3_batch* b, int nc, int w, int waves) {
       b->build_tables(false);
       b->drop_graph();
     }
+  });
+}
+
+int cp3_batch_set_solve_mode(cp3_batch* b, int mode) {
+  return guard([&] {
+    need(b);
+    if (mode != CP3_SOLVE_AUTO && mode != CP3_SOLVE_SHARED && mode != CP3_SOLVE_TABLE)
+      throw Error(CP3_ERR_INVALID, "solve mode must be CP3_SOLVE_AUTO, _SHARED or _TABLE");
+    b->tune_mode = mode;
+    if (!b->jobs.empty()) {
+      b->choose_tuning();
+      b->build_tables(false);
+      b->drop_graph();
+    }
+  });
+}
+
+int cp3_batch_get_tuning(cp3_batch* b, int* nc, int* w, int* waves, int* mode) {
+  return guard([&] {
+    need(b);
+    if (b->jobs.empty()) throw Error(CP3_ERR_STATE, "no jobs set");
+    if (nc) *nc = b->nc;
+    if (w) *w = b->w;
+    if (waves) *waves = b->tune_waves;
+    if (mode) *mode = b->mode;
   });
 }
 

@@ -121,6 +121,55 @@ __device__ __forceinline__ void stage_chunk(SlabStage<P, W>& st, int buf, int c,
   }
 }
 
+// The NC cosines of a wave (perm positions base .. base + NC - 1): table
+// rows, shell counts (0 past the end of the list) and their extremes.
+template <int NC>
+struct CosSet {
+  int cidx[NC];
+  int Mc[NC];
+  bool ok[NC];
+  int Mmin, Mmax;
+};
+
+template <int NC>
+__device__ __forceinline__ void cos_set(const GridDev& g, int base, CosSet<NC>& cs) {
+  cs.Mmax = 0;
+  cs.Mmin = 1 << 30;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const int k = base + j;
+    cs.ok[j] = k < g.nC;
+    cs.cidx[j] = uload(g.perm, cs.ok[j] ? k : g.nC - 1);
+    cs.Mc[j] = cs.ok[j] ? uload(g.M, cs.cidx[j]) : 0;
+    cs.Mmax = cs.Mc[j] > cs.Mmax ? cs.Mc[j] : cs.Mmax;
+    cs.Mmin = cs.Mc[j] < cs.Mmin ? cs.Mc[j] : cs.Mmin;
+  }
+}
+
+// The atmosphere segment and the nine probabilities of the NC trajectories,
+// stored as rows of the [9][nC][nE] output.
+template <class P, int NC>
+__device__ __forceinline__ void finish_store(const Traj<P> (&t)[NC], const Slab<P>& vac,
+                                             const GridDev& g, const CosSet<NC>& cs,
+                                             const SetConst& sc, void* outp, int e,
+                                             bool e_ok) {
+  using D = typename P::D;
+  using F = typename P::F;
+  using O = typename OutT<P>::type;
+  O* out = (O*)outp;
+  const int64_t plane = (int64_t)g.nC * g.nE;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    F prob[9];
+    traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cs.cidx[j]), sc, prob);
+    if (e_ok && cs.ok[j]) {
+      O* dst = out + (int64_t)cs.cidx[j] * g.nE + e;
+#pragma unroll
+      for (int ch = 0; ch < 9; ++ch) dst[ch * plane] = (O)prob[ch];
+    }
+  }
+}
+
 // The trajectories of cosine block sb of grid g for one energy per lane:
 // lane energy index e (stored only if e_ok), the lane's parameter set sc and
 // output array outp ([9][nC][nE]).  sc and outp are wave-uniform for a job's
@@ -130,27 +179,17 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
                                             const SetConst& sc, void* outp,
                                             Stage& st) {
   using D = typename P::D;
-  using F = typename P::F;
-  using O = typename OutT<P>::type;
   const int lane = (int)threadIdx.x % kWave;
   const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
   const D E = (D)g.ev[e_ok ? e : g.nE - 1];
   const D inv = D(1) / E;
 
   const int base = sb * (W * NC);
-  int cidx[NC];
-  int Mc[NC];
-  bool c_ok[NC];
-  int Mmax = 0, Mmin = 1 << 30;
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    const int k = base + wave * NC + j;
-    c_ok[j] = k < g.nC;
-    cidx[j] = uload(g.perm, c_ok[j] ? k : g.nC - 1);
-    Mc[j] = c_ok[j] ? uload(g.M, cidx[j]) : 0;
-    Mmax = Mc[j] > Mmax ? Mc[j] : Mmax;
-    Mmin = Mc[j] < Mmin ? Mc[j] : Mmin;
-  }
+  CosSet<NC> cs;
+  cos_set<NC>(g, base + wave * NC, cs);
+  const int(&cidx)[NC] = cs.cidx;
+  const int(&Mc)[NC] = cs.Mc;
+  const int Mmin = cs.Mmin, Mmax = cs.Mmax;
 
   Traj<P> t[NC];
 #pragma unroll
@@ -217,18 +256,7 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
     __syncthreads();
   }
 
-  O* out = (O*)outp;
-  const int64_t plane = (int64_t)g.nC * g.nE;
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    F prob[9];
-    traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cidx[j]), sc, prob);
-    if (e_ok && c_ok[j]) {
-      O* dst = out + (int64_t)cidx[j] * g.nE + e;
-#pragma unroll
-      for (int ch = 0; ch < 9; ++ch) dst[ch * plane] = (O)prob[ch];
-    }
-  }
+  finish_store<P, NC>(t, vac, g, cs, sc, outp, e, e_ok);
 }
 
 // A work unit of one job: cosine block x one of its own 64-energy tiles.
@@ -344,6 +372,146 @@ __global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_tai
     }
     const TailGroup& tg = groups[lo];
     atmos_tail_unit<P, NC, W>(jobs, tg, u - tg.unit_begin, st);
+  }
+}
+
+// ------------------------------------------------------------ slab table
+//
+// The batch engine can solve every (job, shell, energy) slab once, in a
+// launch of its own, into a table in HBM; the propagation kernels then read
+// each slab with one coalesced row load per wave instead of solving it in
+// every work unit (a work unit shares a solve among its own trajectories, the
+// table among every cosine of the grid).  Table of a job: Slab<P> rows
+// [g.S + 1][energy_tiles(g) * 64], row g.S the vacuum.
+
+// Pre-pass: one wave per (job, shell or vacuum, 64-energy tile).
+template <class P>
+__global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __restrict__ jobs,
+                                                           int njobs, int64_t nwaves) {
+  using D = typename P::D;
+  const int lane = (int)threadIdx.x;
+  for (int64_t u = blockIdx.x; u < nwaves; u += gridDim.x) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].slab_begin <= u)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const AtmosJob& job = jobs[lo];
+    const GridDev& g = job.g;
+    const int et = energy_tiles(g);
+    const int64_t local = u - job.slab_begin;
+    const int i = (int)(local / et);
+    const int e = (int)(local - (int64_t)i * et) * kWave + lane;
+    const D E = (D)g.ev[e < g.nE ? e : g.nE - 1];
+    // g.S >= 1 (an Earth model has rows), so row 0 of rye exists
+    const D rye = (D)uload(g.rye, i < g.S ? i : 0);
+    Slab<P> sl;
+    solve_slab<P>(job.sc, E, D(1) / E, i < g.S ? rye : D(0), sl);
+    ((Slab<P>*)job.slabs)[(int64_t)i * et * kWave + e] = sl;
+  }
+}
+
+// The NC trajectories of cosine block sb (NC cosines) for one energy per
+// lane, reading slab row i of the lane's table column `col` (row stride
+// `row` entries), each row loaded one shell ahead of its use.
+template <class P, int NC>
+__device__ __forceinline__ void table_block(const GridDev& g, int sb, int e, bool e_ok,
+                                            const Slab<P>* col, int64_t row,
+                                            const SetConst& sc, void* outp) {
+  CosSet<NC> cs;
+  cos_set<NC>(g, sb * NC, cs);
+  Traj<P> t[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) traj_init<P>(t[j], cs.Mc[j] == 0);
+  // rows i + 1 <= Mmax <= g.S all exist (row g.S: the vacuum)
+  Slab<P> s = col[0];
+  int i = 0;
+  for (; i < cs.Mmin - 1; ++i) {
+    const Slab<P> nx = col[(int64_t)(i + 1) * row];
+    middle_all<P, NC>(t, s, g, cs.cidx, i);
+    s = nx;
+  }
+  for (; i < cs.Mmax; ++i) {
+    const Slab<P> nx = col[(int64_t)(i + 1) * row];
+    apply_shell<P, NC>(t, s, g, cs.cidx, cs.Mc, i);
+    s = nx;
+  }
+  const Slab<P> vac = col[(int64_t)g.S * row];
+  finish_store<P, NC>(t, vac, g, cs, sc, outp, e, e_ok);
+}
+
+// A work unit of one job in table mode: energy tile x cosine block, the
+// cosine blocks of a tile consecutive so that they share its table rows.
+template <class P, int NC>
+__device__ __forceinline__ void table_unit(const AtmosJob& job, int64_t local) {
+  const int lane = (int)threadIdx.x;
+  const int ncb = (int)cosine_blocks(job.g, NC, 1);
+  const int et = (int)(local / ncb);
+  const int sb = (int)(local - (int64_t)et * ncb);
+  const int e = et * kWave + lane;
+  const int64_t row = (int64_t)energy_tiles(job.g) * kWave;
+  table_block<P, NC>(job.g, sb, e, e < job.g.nE, (const Slab<P>*)job.slabs + e, row,
+                     job.sc, job.out);
+}
+
+// A packed tail unit in table mode: packed tile x cosine block.
+template <class P, int NC>
+__device__ __forceinline__ void tail_table_unit(const AtmosJob* jobs, const TailGroup& tg,
+                                                int64_t local) {
+  const int lane = (int)threadIdx.x;
+  const int ncb = (int)cosine_blocks(tg.g, NC, 1);
+  const int t = (int)(local / ncb);
+  const int sb = (int)(local - (int64_t)t * ncb);
+  const int q = lane / tg.tail;
+  const int slot = t * tg.per + q;
+  const bool ok = q < tg.per && slot < tg.njobs;
+  const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
+  const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
+  const int64_t row = (int64_t)energy_tiles(tg.g) * kWave;
+  table_block<P, NC>(tg.g, sb, e, ok, (const Slab<P>*)job.slabs + e, row, job.sc, job.out);
+}
+
+// Minimum waves per SIMD of the table-mode kernels (0: natural allocation).
+template <class P, int NC>
+struct MinWavesTable {
+  static constexpr int value = 0;
+};
+
+// Table-mode propagation.  Workgroups are dealt to the 8 XCDs round-robin;
+// block b takes unit (b % 8) * per + b / 8, so each XCD walks a contiguous
+// eighth of the unit list and the table rows its units share stay in its
+// own L2 (the grid is a multiple of 8, so a block keeps its XCD share).
+template <class P, int NC>
+__global__ void __launch_bounds__(kWave, (MinWavesTable<P, NC>::value)) atmos_table_kernel(
+    const AtmosJob* __restrict__ jobs, int njobs, int64_t nunits) {
+  const int64_t per = (nunits + 7) / 8;
+  for (int64_t b = blockIdx.x; b < per * 8; b += gridDim.x) {
+    const int64_t u = (b % 8) * per + b / 8;
+    if (u >= nunits) continue;
+    const int j = find_job_atmos(jobs, njobs, u);
+    const AtmosJob& job = jobs[j];
+    table_unit<P, NC>(job, u - job.unit_begin);
+  }
+}
+
+template <class P, int NC>
+__global__ void __launch_bounds__(kWave, (MinWavesTable<P, NC>::value)) tail_table_kernel(
+    const AtmosJob* __restrict__ jobs, const TailGroup* __restrict__ groups, int ngroups,
+    int64_t nunits) {
+  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    int lo = 0, hi = ngroups - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (groups[mid].unit_begin <= u)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const TailGroup& tg = groups[lo];
+    tail_table_unit<P, NC>(jobs, tg, u - tg.unit_begin);
   }
 }
 
@@ -472,6 +640,104 @@ hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* jobs,
     case kFP64: return atmos_tail_p<PolicyF64>(nc, w, jobs, groups, ngroups, nunits, st);
     case kFP32: return atmos_tail_p<PolicyF32>(nc, w, jobs, groups, ngroups, nunits, st);
     case kMixed: return atmos_tail_p<PolicyMixed>(nc, w, jobs, groups, ngroups, nunits, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+#define CP3_NC_CASES(X) X(1) X(2) X(4) X(8)
+
+size_t slab_entry_bytes(int precision) {
+  switch (precision) {
+    case kFP64: return sizeof(Slab<PolicyF64>);
+    case kFP32: return sizeof(Slab<PolicyF32>);
+    case kMixed: return sizeof(Slab<PolicyMixed>);
+    default: return 0;
+  }
+}
+
+hipError_t launch_slab_table(int precision, const AtmosJob* jobs, int njobs,
+                             int64_t nwaves, hipStream_t st) {
+  if (nwaves <= 0 || njobs <= 0) return hipSuccess;
+  if (nwaves > 0x7fffffff) return hipErrorInvalidConfiguration;
+  const dim3 g((unsigned)nwaves), b(kWave);
+  switch (precision) {
+    case kFP64: slab_table_kernel<PolicyF64><<<g, b, 0, st>>>(jobs, njobs, nwaves); break;
+    case kFP32: slab_table_kernel<PolicyF32><<<g, b, 0, st>>>(jobs, njobs, nwaves); break;
+    case kMixed: slab_table_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs, njobs, nwaves); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <class P>
+static hipError_t atmos_table_p(int nc, const AtmosJob* jobs, int njobs, int64_t nunits,
+                                int grid, hipStream_t st) {
+  if (nunits <= 0 || njobs <= 0) return hipSuccess;
+  if (grid <= 0 || grid % 8) return hipErrorInvalidConfiguration;
+#define X(NC_)                                                                  \
+  if (nc == NC_) {                                                              \
+    atmos_table_kernel<P, NC_><<<dim3((unsigned)grid), dim3(kWave), 0, st>>>(    \
+        jobs, njobs, nunits);                                                   \
+    return hipGetLastError();                                                   \
+  }
+  CP3_NC_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_atmos_table(int precision, int nc, const AtmosJob* jobs, int njobs,
+                              int64_t nunits, int grid, hipStream_t st) {
+  switch (precision) {
+    case kFP64: return atmos_table_p<PolicyF64>(nc, jobs, njobs, nunits, grid, st);
+    case kFP32: return atmos_table_p<PolicyF32>(nc, jobs, njobs, nunits, grid, st);
+    case kMixed: return atmos_table_p<PolicyMixed>(nc, jobs, njobs, nunits, grid, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class P>
+static hipError_t tail_table_p(int nc, const AtmosJob* jobs, const TailGroup* groups,
+                               int ngroups, int64_t nunits, hipStream_t st) {
+  if (nunits <= 0 || ngroups <= 0) return hipSuccess;
+  if (nunits > 0x7fffffff) return hipErrorInvalidConfiguration;
+#define X(NC_)                                                                   \
+  if (nc == NC_) {                                                               \
+    tail_table_kernel<P, NC_><<<dim3((unsigned)nunits), dim3(kWave), 0, st>>>(    \
+        jobs, groups, ngroups, nunits);                                          \
+    return hipGetLastError();                                                    \
+  }
+  CP3_NC_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_tail_table(int precision, int nc, const AtmosJob* jobs,
+                             const TailGroup* groups, int ngroups, int64_t nunits,
+                             hipStream_t st) {
+  switch (precision) {
+    case kFP64: return tail_table_p<PolicyF64>(nc, jobs, groups, ngroups, nunits, st);
+    case kFP32: return tail_table_p<PolicyF32>(nc, jobs, groups, ngroups, nunits, st);
+    case kMixed: return tail_table_p<PolicyMixed>(nc, jobs, groups, ngroups, nunits, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <class P>
+static hipError_t table_occ(int nc, int* n) {
+#define X(NC_)                                                                \
+  if (nc == NC_)                                                              \
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, atmos_table_kernel<P, NC_>, \
+                                                        kWave, 0);
+  CP3_NC_CASES(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+hipError_t atmos_table_occupancy(int precision, int nc, int* n) {
+  switch (precision) {
+    case kFP64: return table_occ<PolicyF64>(nc, n);
+    case kFP32: return table_occ<PolicyF32>(nc, n);
+    case kMixed: return table_occ<PolicyMixed>(nc, n);
     default: return hipErrorInvalidValue;
   }
 }

@@ -37,6 +37,11 @@ struct AtmosJob {
   // (energy_tiles), or only the full ones when the last nE % 64 energies
   // run in a packed tail tile (TailGroup)
   int32_t etiles, pad;
+  // Slab-table mode (batch engine): the pre-solved eigen-solutions of this
+  // job, Slab<P> entries [g.S + 1][energy_tiles(g) * 64] (row g.S: the
+  // vacuum), and the job's first wave in the pre-pass launch.
+  void* slabs;
+  int64_t slab_begin;
 };
 
 // Packed tail tiles: the last `tail` = nE % 64 energies of up to `per` =
@@ -67,10 +72,14 @@ struct BeamJob {
 constexpr int kWave = 64;
 
 // Cosine blocks of one grid (w waves x nc cosines each) and 64-energy tiles.
-inline int64_t cosine_blocks(const GridDev& g, int nc, int w) {
+CP3_HD int64_t cosine_blocks(const GridDev& g, int nc, int w) {
   return (g.nC + nc * w - 1) / (nc * w);
 }
-inline int32_t energy_tiles(const GridDev& g) { return (g.nE + kWave - 1) / kWave; }
+CP3_HD int32_t energy_tiles(const GridDev& g) { return (g.nE + kWave - 1) / kWave; }
+// Pre-pass waves of one job's slab table: (shell or vacuum) x energy tile.
+CP3_HD int64_t slab_waves(const GridDev& g) {
+  return (int64_t)(g.S + 1) * energy_tiles(g);
+}
 
 // Work units of one job over all its energy tiles: (cosine block) x (tile).
 inline int64_t atmos_units(const GridDev& g, int nc, int w = 1) {
@@ -91,6 +100,19 @@ hipError_t launch_atmos_single(int precision, int nc, int w, const AtmosJob& job
 hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* d_jobs,
                               int njobs, int64_t nunits, int grid,
                               hipStream_t stream);
+// Slab-table mode: bytes per table entry (sizeof(Slab<P>)), the pre-pass
+// over every job's table (nwaves = sum of slab_waves), and the propagation
+// launches (one wave per workgroup) reading the tables.
+size_t slab_entry_bytes(int precision);
+hipError_t launch_slab_table(int precision, const AtmosJob* d_jobs, int njobs,
+                             int64_t nwaves, hipStream_t stream);
+hipError_t launch_atmos_table(int precision, int nc, const AtmosJob* d_jobs, int njobs,
+                              int64_t nunits, int grid, hipStream_t stream);
+hipError_t launch_tail_table(int precision, int nc, const AtmosJob* d_jobs,
+                             const TailGroup* d_groups, int ngroups, int64_t nunits,
+                             hipStream_t stream);
+hipError_t atmos_table_occupancy(int precision, int nc, int* blocks_per_cu);
+
 // The packed tail tiles of a batch: one workgroup per unit.
 hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* d_jobs,
                              const TailGroup* d_groups, int ngroups, int64_t nunits,

@@ -89,7 +89,16 @@ def grid_stage_calls(M: np.ndarray, n_e: int, nc: int, w: int = 1) -> dict:
     }
 
 
-def batch_counts(wl, precision: str, nc: int, w: int = 1) -> dict:
+def table_stage_calls(M: np.ndarray, n_e: int, n_shells: int) -> dict:
+    """Stage calls of one atmospheric job in slab-table mode: the pre-pass
+    solves every shell of the Earth model and the vacuum once per energy."""
+    calls = grid_stage_calls(M, n_e, 1, 1)
+    calls["solve"] = (n_shells + 1) * n_e
+    return calls
+
+
+def batch_counts(wl, precision: str, nc: int, w: int = 1,
+                 mode: str = "shared") -> dict:
     """Total counted work of one launch over an AtmosBatch."""
     ops = stage_ops(precision)
     tot = {k: 0.0 for k in KEYS}
@@ -98,7 +107,10 @@ def batch_counts(wl, precision: str, nc: int, w: int = 1) -> dict:
         key = (j.grid, len(j.cosv), len(j.ev))
         if key not in cache:
             M = max_layers(j.cosv, wl.earth)
-            cache[key] = grid_stage_calls(M, len(j.ev), nc, w)
+            if mode == "table":
+                cache[key] = table_stage_calls(M, len(j.ev), len(wl.earth.radii))
+            else:
+                cache[key] = grid_stage_calls(M, len(j.ev), nc, w)
         calls = cache[key]
         for st, n in calls.items():
             for k in KEYS:

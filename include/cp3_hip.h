@@ -209,8 +209,26 @@ int cp3_batch_update_jobs(cp3_batch* b, const cp3_job* jobs, int n_jobs);
  * depend on it (bitwise). */
 int cp3_batch_set_tuning(cp3_batch* b, int cosine_block, int group_waves,
                          int workgroups_per_cu);
-/* Times every candidate tuning on the current job list (reps launches each)
- * and keeps the fastest; reports it (any pointer may be NULL). */
+/* Where the per-(energy, shell) eigen-solutions come from:
+ *   CP3_SOLVE_SHARED - solved inside the propagation kernel, each solve
+ *                      shared by the cosine block x group waves of a work
+ *                      unit through LDS;
+ *   CP3_SOLVE_TABLE  - solved once per (job, shell, energy) by a pre-pass
+ *                      into a table in device memory (about
+ *                      (shells + 1) x energies x 40 bytes per job, mixed
+ *                      precision), read by every cosine; group_waves is 1;
+ *   CP3_SOLVE_AUTO   - the table when the tables fit in 16 GiB (default).
+ * Results do not depend on the mode (bitwise). */
+#define CP3_SOLVE_AUTO 0
+#define CP3_SOLVE_SHARED 1
+#define CP3_SOLVE_TABLE 2
+int cp3_batch_set_solve_mode(cp3_batch* b, int mode);
+/* The tuning in effect for the current job list (any pointer may be NULL). */
+int cp3_batch_get_tuning(cp3_batch* b, int* cosine_block, int* group_waves,
+                         int* workgroups_per_cu, int* solve_mode);
+/* Times every candidate tuning on the current job list (reps launches each),
+ * over both solve modes unless one was set, and keeps the fastest; reports
+ * it (any pointer may be NULL; cp3_batch_get_tuning gives the mode). */
 int cp3_batch_autotune(cp3_batch* b, int reps, int* cosine_block,
                        int* group_waves, int* workgroups_per_cu);
 /* Capture the launch into a hipGraph on the next run and replay it after. */

@@ -54,9 +54,10 @@ def small_mix():
     return earth, grids, jobs, sets, beam_ev
 
 
-def build(precision, nc=0, group=0, waves=0):
+def build(precision, nc=0, group=0, waves=0, mode="auto"):
     earth, grids, jobs, sets, beam_ev = small_mix()
     b = Batch(precision)
+    b.set_solve_mode(mode)
     gid = {k: b.add_atmos_grid(c, e, earth, 22.0) for k, (c, e) in grids.items()}
     bg = b.add_beam_grid(beam_ev, 2.6, 1300.0)
     cj = [make_job(ATMOS, gid[j.grid], j.params, j.nutype) for j in jobs]
@@ -68,10 +69,15 @@ def build(precision, nc=0, group=0, waves=0):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("tuning", [(1, 1, 0), (4, 1, -1), (8, 2, 2),
-                                    (2, 4, 0), (4, 4, -1)])
+@pytest.mark.parametrize("tuning", [(1, 1, 0, "shared"), (4, 1, -1, "shared"),
+                                    (8, 2, 2, "shared"), (2, 4, 0, "shared"),
+                                    (4, 4, -1, "shared"), (1, 0, 0, "table"),
+                                    (2, 0, -1, "table"), (4, 0, 2, "table"),
+                                    (8, 0, -1, "table")])
 def test_batch_equals_single(precision, tuning):
     b, earth, jobs, sets, beam_ev = build(precision, *tuning)
+    assert b.tuning()[3] == tuning[3]
+    assert b.tuning()[0] == tuning[0]
     b.run()
     b.wait()
     for i, job in enumerate(jobs):
@@ -83,8 +89,9 @@ def test_batch_equals_single(precision, tuning):
     b.close()
 
 
-def test_graph_replays_equal_eager():
-    b, earth, jobs, sets, beam_ev = build("mixed")
+@pytest.mark.parametrize("mode", ["shared", "table"])
+def test_graph_replays_equal_eager(mode):
+    b, earth, jobs, sets, beam_ev = build("mixed", mode=mode)
     b.run()
     b.wait()
     eager = [b.result(i) for i in range(len(jobs) + 2)]
@@ -132,6 +139,8 @@ def test_autotune_picks_valid():
     b, *_ = build("mixed")
     nc, group, waves = b.autotune(reps=1)
     assert nc in (1, 2, 4, 8) and group in (1, 2, 4)
+    assert b.tuning()[:3] == (nc, group, waves)
+    assert b.tuning()[3] in ("shared", "table")
     b.run()
     b.wait()
     assert b.last_ms() > 0
@@ -157,7 +166,9 @@ def test_batch_errors():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("tuning", [(1, 1, 0), (4, 4, -1), (8, 2, 0)])
+@pytest.mark.parametrize("tuning", [(1, 1, 0, "shared"), (4, 4, -1, "shared"),
+                                    (8, 2, 0, "shared"), (1, 0, 0, "table"),
+                                    (4, 0, -1, "table"), (8, 0, 0, "table")])
 def test_packed_tails_edge_cases(precision, tuning):
     """Jobs sharing a grid run their last nE % 64 energies in packed tail
     tiles: a grid with fewer than 64 energies (every lane in tails, several
@@ -174,10 +185,12 @@ def test_packed_tails_edge_cases(precision, tuning):
     plan += [("d", sets[0], 0), ("d", sets[1], 1), ("e", sets[2], 0),
              ("e", sets[3], 1), ("f", sets[0], 1)]
     b = Batch(precision)
+    b.set_solve_mode(tuning[3])
     gid = {k: b.add_atmos_grid(c, e, earth, 22.0) for k, (c, e) in grids.items()}
     jobs = [W.AtmosJob(p, grids[g][0], grids[g][1], nt, grid=g) for g, p, nt in plan]
     b.set_jobs([make_job(ATMOS, gid[j.grid], j.params, j.nutype) for j in jobs])
-    b.set_tuning(*tuning)
+    b.set_tuning(*tuning[:3])
+    assert b.tuning()[3] == tuning[3]
     b.run()
     b.wait()
     for i, job in enumerate(jobs):
