@@ -169,9 +169,8 @@ struct cp3_batch {
       const cp3::GridDev g = agrids[(size_t)jobs[j].grid]->dev();
       slab_off[a] = off;
       ajobs[a].slab_begin = wv;
-      const int64_t waves = cp3::slab_waves(g);
-      off = align_up(off + (size_t)waves * cp3::kWave * entry);
-      wv += waves;
+      off = align_up(off + (size_t)cp3::slab_entries(g) * entry);
+      wv += cp3::slab_waves(g);
     }
     slab_total = off;
     s_waves = wv;
