@@ -58,7 +58,7 @@ struct cp3_batch {
   int64_t a_units = 0, t_units = 0, b_units = 0, evals = 0;
   size_t out_total = 0;
   // slab tables (CP3_SOLVE_TABLE): per atmospheric job its byte offset in
-  // d_slabs, and the pre-pass size
+  // d_slabs, and the pre-pass grid width (the largest job's waves)
   std::vector<size_t> slab_off;
   size_t slab_total = 0;
   int64_t s_waves = 0;
@@ -157,7 +157,7 @@ struct cp3_batch {
   }
 
   // Slab-table layout: job a's table at slab_off[a], [S + 1][tiles * 64]
-  // entries; its pre-pass waves from slab_begin.
+  // entries.
   void plan_slabs() {
     const size_t entry = cp3::slab_entry_bytes(precision);
     slab_off.assign(ajobs.size(), 0);
@@ -168,19 +168,21 @@ struct cp3_batch {
       const size_t a = (size_t)slot[j];
       const cp3::GridDev g = agrids[(size_t)jobs[j].grid]->dev();
       slab_off[a] = off;
-      ajobs[a].slab_begin = wv;
-      off = align_up(off + (size_t)cp3::slab_entries(g) * entry);
-      wv += cp3::slab_waves(g);
+      const int64_t waves = cp3::slab_waves(g);
+      off = align_up(off + (size_t)waves * cp3::kWave * entry);
+      wv = std::max(wv, waves);
     }
     slab_total = off;
     s_waves = wv;
   }
 
-  // Table mode unless the tables would take more than kTableMaxBytes.
+  // Table mode unless the tables would take more than kTableMaxBytes (or the
+  // pre-pass grid would have more than 65535 rows of jobs).
   static constexpr size_t kTableMaxBytes = (size_t)16 << 30;
   int auto_mode() const {
-    return slab_total > 0 && slab_total <= kTableMaxBytes ? CP3_SOLVE_TABLE
-                                                          : CP3_SOLVE_SHARED;
+    return slab_total > 0 && slab_total <= kTableMaxBytes && ajobs.size() <= 65535
+               ? CP3_SOLVE_TABLE
+               : CP3_SOLVE_SHARED;
   }
 
   // Atmospheric jobs sharing a grid whose nE % 64 leaves room for two or
@@ -279,7 +281,8 @@ struct cp3_batch {
   // Unit prefix sums for the chosen cosine block, and grid sizes.
   void choose_tuning() {
     mode = tune_mode ? tune_mode : auto_mode();
-    if (mode == CP3_SOLVE_TABLE && slab_total == 0) mode = CP3_SOLVE_SHARED;
+    if (mode == CP3_SOLVE_TABLE && (slab_total == 0 || ajobs.size() > 65535))
+      mode = CP3_SOLVE_SHARED;
     nc = tune_nc ? tune_nc : (mode == CP3_SOLVE_TABLE ? 4 : auto_nc());
     // the table replaces the waves' shared solves: one wave per workgroup
     w = mode == CP3_SOLVE_TABLE ? 1 : (tune_w ? tune_w : 4);

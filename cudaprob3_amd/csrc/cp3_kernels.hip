@@ -40,20 +40,20 @@ __device__ __forceinline__ T uload(const T* p, int64_t i) {
 // Shell i as a middle shell of all NC trajectories of a wave (i < Mmin - 1,
 // Mmin the smallest shell count among them): branch-free, so the NC
 // independent update chains interleave.
-template <class P, int NC, class S>
-__device__ __forceinline__ void middle_all(Traj<P> (&t)[NC], const S& s,
+template <class P, int NC>
+__device__ __forceinline__ void middle_all(Traj<P> (&t)[NC], const Slab<P>& s,
                                            const GridDev& g, const int (&cidx)[NC],
                                            int i) {
   using D = typename P::D;
 #pragma unroll
   for (int j = 0; j < NC; ++j)
-    shell_middle<P>(t[j], s, (D)uload(g.seg, (int64_t)cidx[j] * g.S + i));
+    traj_middle<P>(t[j], s, (D)uload(g.seg, (int64_t)cidx[j] * g.S + i));
 }
 
 // Shell i for trajectories that may end at or before it: a middle shell of
 // those with more shells, the innermost of those with exactly i + 1.
-template <class P, int NC, class S>
-__device__ __forceinline__ void apply_shell(Traj<P> (&t)[NC], const S& s,
+template <class P, int NC>
+__device__ __forceinline__ void apply_shell(Traj<P> (&t)[NC], const Slab<P>& s,
                                             const GridDev& g, const int (&cidx)[NC],
                                             const int (&Mc)[NC], int i) {
   using D = typename P::D;
@@ -61,9 +61,9 @@ __device__ __forceinline__ void apply_shell(Traj<P> (&t)[NC], const S& s,
   for (int j = 0; j < NC; ++j) {
     const D L = (D)uload(g.seg, (int64_t)cidx[j] * g.S + i);
     if (i < Mc[j] - 1) {
-      shell_middle<P>(t[j], s, L);
+      traj_middle<P>(t[j], s, L);
     } else if (i == Mc[j] - 1) {
-      shell_inner<P>(t[j], s, L);
+      traj_inner<P>(t[j], s, L);
     }
   }
 }
@@ -379,89 +379,59 @@ __global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_tai
 //
 // The batch engine can solve every (job, shell, energy) slab once, in a
 // launch of its own, into a table in HBM; the propagation kernels then read
-// each entry with one coalesced row load per wave instead of solving it in
+// each slab with one coalesced row load per wave instead of solving it in
 // every work unit (a work unit shares a solve among its own trajectories, the
-// table among every cosine of the grid).  Table of a job: entries
-// TabEntryOf<P> (eigenbasis rotations, or rank-one slabs for FP32) in rows
+// table among every cosine of the grid).  Table of a job: Slab<P> rows
 // [g.S + 1][energy_tiles(g) * 64], row g.S the vacuum.
 
-// Pre-pass: one wave per (job, 64-energy tile) walks the shells in order,
-// carrying the previous shell's eigenvectors for the rotations.
+// Pre-pass: one wave per (job, shell or vacuum, 64-energy tile), on a 2-D
+// grid (y: job, x: the job's shell x tile) so that a wave finds its job
+// without a search; waves past a smaller job's count exit at once.
 template <class P>
-__global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __restrict__ jobs,
-                                                           int njobs, int64_t nwaves) {
+__global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __restrict__ jobs) {
   using D = typename P::D;
-  using T = typename TabEntryOf<P>::type;
   const int lane = (int)threadIdx.x;
-  for (int64_t u = blockIdx.x; u < nwaves; u += gridDim.x) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].slab_begin <= u)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const AtmosJob& job = jobs[lo];
-    const GridDev& g = job.g;
-    const int et = energy_tiles(g);
-    const int e = (int)(u - job.slab_begin) * kWave + lane;
-    const int64_t row = (int64_t)et * kWave;
-    T* col = (T*)job.slabs + e;
-    const D E = (D)g.ev[e < g.nE ? e : g.nE - 1];
-    const D inv = D(1) / E;
-    if constexpr (P::kRot) {
-      D Vp[9] = {D(1), D(0), D(0), D(0), D(1), D(0), D(0), D(0), D(1)}, V[9];
-      for (int i = 0; i < g.S; ++i) {
-        Rot<P> r;
-        solve_rot<P>(job.sc, E, inv, (D)uload(g.rye, i), Vp, V, r);
-        col[i * row] = r;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Vp[k] = V[k];
-      }
-      const D I[9] = {D(1), D(0), D(0), D(0), D(1), D(0), D(0), D(0), D(1)};
-      Rot<P> r;
-      solve_rot<P>(job.sc, E, inv, D(0), I, V, r, false);
-      col[(int64_t)g.S * row] = r;
-    } else {
-      for (int i = 0; i <= g.S; ++i) {
-        Slab<P> sl;
-        solve_slab<P>(job.sc, E, inv, i < g.S ? (D)uload(g.rye, i < g.S ? i : 0) : D(0), sl);
-        col[i * row] = sl;
-      }
-    }
-  }
+  const AtmosJob& job = jobs[blockIdx.y];
+  const GridDev& g = job.g;
+  const int64_t u = blockIdx.x;
+  if (u >= slab_waves(g)) return;
+  const int et = energy_tiles(g);
+  const int i = (int)(u / et);
+  const int e = (int)(u - (int64_t)i * et) * kWave + lane;
+  const D E = (D)g.ev[e < g.nE ? e : g.nE - 1];
+  // g.S >= 1 (an Earth model has rows), so row 0 of rye exists
+  const D rye = (D)uload(g.rye, i < g.S ? i : 0);
+  Slab<P> sl;
+  solve_slab<P>(job.sc, E, D(1) / E, i < g.S ? rye : D(0), sl);
+  ((Slab<P>*)job.slabs)[(int64_t)i * et * kWave + e] = sl;
 }
 
 // The NC trajectories of cosine block sb (NC cosines) for one energy per
-// lane, reading row i of the lane's table column `col` (row stride `row`
-// entries), each row loaded one shell ahead of its use.
+// lane, reading slab row i of the lane's table column `col` (row stride
+// `row` entries), each row loaded one shell ahead of its use.
 template <class P, int NC>
 __device__ __forceinline__ void table_block(const GridDev& g, int sb, int e, bool e_ok,
-                                            const typename TabEntryOf<P>::type* col,
-                                            int64_t row, const SetConst& sc, void* outp) {
-  using T = typename TabEntryOf<P>::type;
+                                            const Slab<P>* col, int64_t row,
+                                            const SetConst& sc, void* outp) {
   CosSet<NC> cs;
   cos_set<NC>(g, sb * NC, cs);
   Traj<P> t[NC];
 #pragma unroll
   for (int j = 0; j < NC; ++j) traj_init<P>(t[j], cs.Mc[j] == 0);
   // rows i + 1 <= Mmax <= g.S all exist (row g.S: the vacuum)
-  T s = col[0];
+  Slab<P> s = col[0];
   int i = 0;
-#pragma unroll 2
   for (; i < cs.Mmin - 1; ++i) {
-    const T nx = col[(int64_t)(i + 1) * row];
+    const Slab<P> nx = col[(int64_t)(i + 1) * row];
     middle_all<P, NC>(t, s, g, cs.cidx, i);
     s = nx;
   }
   for (; i < cs.Mmax; ++i) {
-    const T nx = col[(int64_t)(i + 1) * row];
+    const Slab<P> nx = col[(int64_t)(i + 1) * row];
     apply_shell<P, NC>(t, s, g, cs.cidx, cs.Mc, i);
     s = nx;
   }
-  Slab<P> vac;
-  vacuum_slab<P>(col[(int64_t)g.S * row], vac);
+  const Slab<P> vac = col[(int64_t)g.S * row];
   finish_store<P, NC>(t, vac, g, cs, sc, outp, e, e_ok);
 }
 
@@ -475,9 +445,8 @@ __device__ __forceinline__ void table_unit(const AtmosJob& job, int64_t local) {
   const int sb = (int)(local - (int64_t)et * ncb);
   const int e = et * kWave + lane;
   const int64_t row = (int64_t)energy_tiles(job.g) * kWave;
-  using T = typename TabEntryOf<P>::type;
-  table_block<P, NC>(job.g, sb, e, e < job.g.nE, (const T*)job.slabs + e, row, job.sc,
-                     job.out);
+  table_block<P, NC>(job.g, sb, e, e < job.g.nE, (const Slab<P>*)job.slabs + e, row,
+                     job.sc, job.out);
 }
 
 // A packed tail unit in table mode: packed tile x cosine block.
@@ -494,21 +463,15 @@ __device__ __forceinline__ void tail_table_unit(const AtmosJob* jobs, const Tail
   const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
   const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
   const int64_t row = (int64_t)energy_tiles(tg.g) * kWave;
-  using T = typename TabEntryOf<P>::type;
-  table_block<P, NC>(tg.g, sb, e, ok, (const T*)job.slabs + e, row, job.sc, job.out);
+  table_block<P, NC>(tg.g, sb, e, ok, (const Slab<P>*)job.slabs + e, row, job.sc, job.out);
 }
 
-// Minimum waves per SIMD of the table-mode kernels (0: natural allocation):
-// two for mixed NC = 8, whose natural allocation just exceeds 256 VGPRs
-// (the cap costs it no spills; FP64 NC = 4 would spill heavily).
+// Minimum waves per SIMD of the table-mode kernels (0: natural allocation).
 template <class P, int NC>
 struct MinWavesTable {
   static constexpr int value = 0;
 };
-template <>
-struct MinWavesTable<PolicyMixed, 8> {
-  static constexpr int value = 2;
-};
+
 
 // Table-mode propagation.  Workgroups are dealt to the 8 XCDs round-robin;
 // block b takes unit (b % 8) * per + b / 8, so each XCD walks a contiguous
@@ -678,22 +641,22 @@ hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* jobs,
 
 size_t slab_entry_bytes(int precision) {
   switch (precision) {
-    case kFP64: return sizeof(TabEntryOf<PolicyF64>::type);
-    case kFP32: return sizeof(TabEntryOf<PolicyF32>::type);
-    case kMixed: return sizeof(TabEntryOf<PolicyMixed>::type);
+    case kFP64: return sizeof(Slab<PolicyF64>);
+    case kFP32: return sizeof(Slab<PolicyF32>);
+    case kMixed: return sizeof(Slab<PolicyMixed>);
     default: return 0;
   }
 }
 
 hipError_t launch_slab_table(int precision, const AtmosJob* jobs, int njobs,
-                             int64_t nwaves, hipStream_t st) {
-  if (nwaves <= 0 || njobs <= 0) return hipSuccess;
-  if (nwaves > 0x7fffffff) return hipErrorInvalidConfiguration;
-  const dim3 g((unsigned)nwaves), b(kWave);
+                             int64_t max_waves, hipStream_t st) {
+  if (max_waves <= 0 || njobs <= 0) return hipSuccess;
+  if (max_waves > 0x7fffffff || njobs > 65535) return hipErrorInvalidConfiguration;
+  const dim3 g((unsigned)max_waves, (unsigned)njobs), b(kWave);
   switch (precision) {
-    case kFP64: slab_table_kernel<PolicyF64><<<g, b, 0, st>>>(jobs, njobs, nwaves); break;
-    case kFP32: slab_table_kernel<PolicyF32><<<g, b, 0, st>>>(jobs, njobs, nwaves); break;
-    case kMixed: slab_table_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs, njobs, nwaves); break;
+    case kFP64: slab_table_kernel<PolicyF64><<<g, b, 0, st>>>(jobs); break;
+    case kFP32: slab_table_kernel<PolicyF32><<<g, b, 0, st>>>(jobs); break;
+    case kMixed: slab_table_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -38,10 +38,9 @@ struct AtmosJob {
   // run in a packed tail tile (TailGroup)
   int32_t etiles, pad;
   // Slab-table mode (batch engine): the pre-solved eigen-solutions of this
-  // job, TabEntryOf<P> entries [g.S + 1][energy_tiles(g) * 64] (row g.S:
-  // the vacuum), and the job's first wave in the pre-pass launch.
+  // job, Slab<P> entries [g.S + 1][energy_tiles(g) * 64] (row g.S: the
+  // vacuum).
   void* slabs;
-  int64_t slab_begin;
 };
 
 // Packed tail tiles: the last `tail` = nE % 64 energies of up to `per` =
@@ -76,12 +75,9 @@ CP3_HD int64_t cosine_blocks(const GridDev& g, int nc, int w) {
   return (g.nC + nc * w - 1) / (nc * w);
 }
 CP3_HD int32_t energy_tiles(const GridDev& g) { return (g.nE + kWave - 1) / kWave; }
-// Pre-pass waves of one job's slab table: one per energy tile (each walks
-// the g.S + 1 rows).
-CP3_HD int64_t slab_waves(const GridDev& g) { return energy_tiles(g); }
-// Entries of one job's slab table.
-CP3_HD int64_t slab_entries(const GridDev& g) {
-  return (int64_t)(g.S + 1) * energy_tiles(g) * 64;
+// Pre-pass waves of one job's slab table: (shell or vacuum) x energy tile.
+CP3_HD int64_t slab_waves(const GridDev& g) {
+  return (int64_t)(g.S + 1) * energy_tiles(g);
 }
 
 // Work units of one job over all its energy tiles: (cosine block) x (tile).
@@ -103,12 +99,13 @@ hipError_t launch_atmos_single(int precision, int nc, int w, const AtmosJob& job
 hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* d_jobs,
                               int njobs, int64_t nunits, int grid,
                               hipStream_t stream);
-// Slab-table mode: bytes per table entry (sizeof(TabEntryOf<P>)), the pre-pass
-// over every job's table (nwaves = sum of slab_waves), and the propagation
-// launches (one wave per workgroup) reading the tables.
+// Slab-table mode: bytes per table entry (sizeof(Slab<P>)), the pre-pass
+// over every job's table (a max_waves x njobs grid, max_waves the largest
+// slab_waves; at most 65535 jobs), and the propagation launches (one wave
+// per workgroup) reading the tables.
 size_t slab_entry_bytes(int precision);
 hipError_t launch_slab_table(int precision, const AtmosJob* d_jobs, int njobs,
-                             int64_t nwaves, hipStream_t stream);
+                             int64_t max_waves, hipStream_t stream);
 hipError_t launch_atmos_table(int precision, int nc, const AtmosJob* d_jobs, int njobs,
                               int64_t nunits, int grid, hipStream_t stream);
 hipError_t launch_tail_table(int precision, int nc, const AtmosJob* d_jobs,

@@ -48,8 +48,6 @@
 #include <math.h>
 #include <stdint.h>
 
-#include <type_traits>
-
 namespace cp3 {
 
 // constants.hpp:28-40 and physics.hpp:444.
@@ -87,23 +85,17 @@ struct SetConst {
 
 // ---------------------------------------------------------------- policies
 
-// kRot: the slab-table form uses the eigenbasis chain (Rot) rather than
-// rank-one slabs; it needs D-precision eigenvectors to form the rotations
-// (with FP32 ones the chain loses accuracy the rank-one form keeps).
 struct PolicyF64 {
   using D = double;
   using F = double;
-  static constexpr bool kRot = true;
 };
 struct PolicyF32 {
   using D = float;
   using F = float;
-  static constexpr bool kRot = false;
 };
 struct PolicyMixed {
   using D = double;
   using F = float;
-  static constexpr bool kRot = true;
 };
 
 // Scalar wrappers, one overload per type, so the host build can count
@@ -227,19 +219,6 @@ CP3_HD void m_cispi_m1(double f, double* wr, double* wi) {
   *wi = a * b;
 }
 
-// e^{2 pi i f} for |f| <= 1/2 from the same polynomials:
-// (cos 2 pi f, sin 2 pi f) = (1 - a^2, a b).
-CP3_HD void m_cispi(float f, float* c, float* s) {
-  float wr;
-  m_cispi_m1(f, &wr, s);
-  *c = 1.0f + wr;
-}
-CP3_HD void m_cispi(double f, double* c, double* s) {
-  double wr;
-  m_cispi_m1(f, &wr, s);
-  *c = 1.0 + wr;
-}
-
 // Layer phases in revolutions: om is in revolutions per km, the phase
 // q = om * L is reduced in D to f = q - rint(q) in [-1/2, 1/2], summed into
 // acc (det Y = e^{2 pi i acc}), and e^{2 pi i f} - 1 is evaluated in F.
@@ -261,19 +240,9 @@ struct PhaseRev {
   }
   // e^{2 pi i acc}
   CP3_HD static void unit(D acc, F& cr, F& ci) {
-    m_cispi(F(acc - m_rint(acc)), &cr, &ci);
-  }
-  // e^{2 pi i om L} itself, summed into acc like eval
-  CP3_HD static void evalc(D om, D L, D& acc, F& cr, F& ci) {
-    const D q = om * L;
-    const D f = q - m_rint(q);
-    acc += f;
-    if (sizeof(D) == 4) acc -= m_rint(acc);
-    m_cispi(F(f), &cr, &ci);
-  }
-  CP3_HD static void evalc(D om, D L, F& cr, F& ci) {
-    const D q = om * L;
-    m_cispi(F(q - m_rint(q)), &cr, &ci);
+    F wr;
+    m_cispi_m1(F(acc - m_rint(acc)), &wr, &ci);
+    cr = wr + F(1);
   }
 };
 
@@ -344,13 +313,10 @@ CP3_HD void matter_eigs(const SetConst& sc, D fac, D lam[3]) {
     lam[k] = amp * (D(sc.ra[k]) * cs + D(sc.rb[k]) * sn) + shift;
 }
 
-// The eigen-solution of one slab of electron density rhoYe at energy E
-// (invE = 1/E) in D: unit eigenvectors of the far and near eigenvalues and
-// their phase rates relative to the reference (revolutions per km).
-template <class P, class V>
-CP3_HD void eigen_frame(const SetConst& sc, typename P::D E, typename P::D invE,
-                        typename P::D rhoYe, V va[3], V vb[3], typename P::D& oma,
-                        typename P::D& omb) {
+// Solves one slab of electron density rhoYe at energy E (invE = 1/E).
+template <class P>
+CP3_HD void solve_slab(const SetConst& sc, typename P::D E,
+                       typename P::D invE, typename P::D rhoYe, Slab<P>& s) {
   using D = typename P::D;
   const D fac = D(sc.sgn) * ((D(kGf2) * E) * rhoYe);
   D lam[3];
@@ -371,86 +337,11 @@ CP3_HD void eigen_frame(const SetConst& sc, typename P::D E, typename P::D invE,
   const D lr = p01 ? lam[0] : lam[2];
   const D ln = (p01 || p12) ? lam[1] : lam[0];
   const D lf = p01 ? lam[2] : (p12 ? lam[0] : lam[1]);
-  eigvec<D>(H, H2, lf, lr, ln, va);
-  eigvec<D>(H, H2, ln, lr, lf, vb);
+  eigvec<D>(H, H2, lf, lr, ln, s.va);
+  eigvec<D>(H, H2, ln, lr, lf, s.vb);
   const D k = D(-kLoE * Phase<P>::kOmScale) * invE;
-  oma = k * (lf - lr);
-  omb = k * (ln - lr);
-}
-
-// Solves one slab of electron density rhoYe at energy E (invE = 1/E).
-template <class P>
-CP3_HD void solve_slab(const SetConst& sc, typename P::D E,
-                       typename P::D invE, typename P::D rhoYe, Slab<P>& s) {
-  eigen_frame<P>(sc, E, invE, rhoYe, s.va, s.vb, s.oma, s.omb);
-}
-
-// ------------------------------------------------- eigenbasis (table) form
-//
-// With orthonormal eigenvectors V_i = [va vb vr] of shell i (vr = va x vb,
-// so det V_i = 1), T_i = V_i D_i V_i^T with D_i = diag(e^{i psi_a},
-// e^{i psi_b}, 1) after the reference phase is removed.  Y' = V_i^T Y then
-// evolves as Y' <- D_i R_i Y' with the real rotation R_i = V_i^T V_{i-1}
-// (V_{-1} = I): per column a real 3x3 product and two phase factors, in
-// place of two rank-one updates.  Y' is unitary with det Y' = det Y, and
-// Z = Y^T T_M Y = (R_M Y')^T D_M (R_M Y').  vb is orthogonalised against va
-// in D first: for a near-degenerate pair vb is ill-conditioned, and only an
-// exactly orthonormal V keeps its error multiplied by the small phase
-// difference of the pair.
-template <class P>
-struct Rot {
-  typename P::D oma, omb;  // phase rates of va, vb (revolutions per km)
-  typename P::F r[9];      // K = R - I row-major (R itself for a chain's start)
-};
-
-// Shell i of a chain: solves it, orthonormalises, and forms R = V^T Vp from
-// the previous shell's eigenvectors Vp (columns va vb vr, D[9]); V receives
-// this shell's.  Stored as K = R - I (minus_identity), formed in D: neighbouring
-// shells have close eigenvectors, so K is small and its rounding to F costs
-// far less than that of R.  With Vp = I and R itself stored, the rows are
-// va, vb, vr.
-template <class P>
-CP3_HD void solve_rot(const SetConst& sc, typename P::D E, typename P::D invE,
-                      typename P::D rhoYe, const typename P::D Vp[9],
-                      typename P::D V[9], Rot<P>& out, bool minus_identity = true) {
-  using D = typename P::D;
-  D va[3], vb[3];
-  eigen_frame<P>(sc, E, invE, rhoYe, va, vb, out.oma, out.omb);
-  const D d = va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2];
-#pragma unroll
-  for (int m = 0; m < 3; ++m) vb[m] = vb[m] - d * va[m];
-  const D n = m_rsqrt(vb[0] * vb[0] + vb[1] * vb[1] + vb[2] * vb[2]);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    V[m] = va[m];
-    V[3 + m] = vb[m] * n;
-  }
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const int p = (m + 1) % 3, q = (m + 2) % 3;
-    V[6 + m] = V[p] * V[3 + q] - V[q] * V[3 + p];
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      D rkm = V[3 * k] * Vp[3 * m] + V[3 * k + 1] * Vp[3 * m + 1] + V[3 * k + 2] * Vp[3 * m + 2];
-      if (minus_identity && k == m) rkm = rkm - D(1);
-      out.r[3 * k + m] = (typename P::F)rkm;
-    }
-}
-
-// The far and near eigenvectors and phase rates of a chain's vacuum entry
-// (formed against Vp = I, R itself stored): rows 0 and 1 of its R.
-template <class P>
-CP3_HD void rot_slab(const Rot<P>& r, Slab<P>& s) {
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    s.va[m] = r.r[m];
-    s.vb[m] = r.r[3 + m];
-  }
-  s.oma = r.oma;
-  s.omb = r.omb;
+  s.oma = k * (lf - lr);
+  s.omb = k * (ln - lr);
 }
 
 // ---------------------------------------------------- complex helpers
@@ -620,141 +511,6 @@ CP3_HD void traj_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
   t.y1.re[0] = zr[3]; t.y1.im[0] = zi[3];
   t.y1.re[1] = zr[4]; t.y1.im[1] = zi[4];
   t.y1.re[2] = zr[5]; t.y1.im[2] = zi[5];
-}
-
-// u = R y = y + K y for one column (K = R - I): a chain of fused
-// multiply-adds on y_k.
-template <class F>
-CP3_HD void rot_apply(const F k9[9], const C3<F>& y, F ur[3], F ui[3]) {
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    ur[k] = y.re[k] + k9[3 * k] * y.re[0];
-    ur[k] = ur[k] + k9[3 * k + 1] * y.re[1];
-    ur[k] = ur[k] + k9[3 * k + 2] * y.re[2];
-    ui[k] = y.im[k] + k9[3 * k] * y.im[0];
-    ui[k] = ui[k] + k9[3 * k + 1] * y.im[1];
-    ui[k] = ui[k] + k9[3 * k + 2] * y.im[2];
-  }
-}
-
-// y <- D R y for one column: the rotation, then the phase factors
-// 1 + wa and 1 + wb on components 0 and 1, applied as u + w u so that a
-// small phase keeps its relative precision.
-template <class F>
-CP3_HD void rot_column(const F r[9], C3<F>& y, F war, F wai, F wbr, F wbi) {
-  F ur[3], ui[3];
-  rot_apply(r, y, ur, ui);
-  y.re[0] = ur[0] + war * ur[0];
-  y.re[0] = y.re[0] - wai * ui[0];
-  y.im[0] = ui[0] + war * ui[0];
-  y.im[0] = y.im[0] + wai * ur[0];
-  y.re[1] = ur[1] + wbr * ur[1];
-  y.re[1] = y.re[1] - wbi * ui[1];
-  y.im[1] = ui[1] + wbr * ui[1];
-  y.im[1] = y.im[1] + wbi * ur[1];
-  y.re[2] = ur[2];
-  y.im[2] = ui[2];
-}
-
-// A middle shell in the eigenbasis form: Y' <- D R Y'.
-template <class P>
-CP3_HD void traj_middle_rot(Traj<P>& t, const Rot<P>& r, typename P::D L) {
-  using F = typename P::F;
-  F war, wai, wbr, wbi;
-  Phase<P>::eval(r.oma, L, t.acc, war, wai);
-  Phase<P>::eval(r.omb, L, t.acc, wbr, wbi);
-  rot_column(r.r, t.y0, war, wai, wbr, wbi);
-  rot_column(r.r, t.y1, war, wai, wbr, wbi);
-}
-
-// The innermost shell in the eigenbasis form: U = R [y0 y1 y2] and
-// Z = U^T D U = sum_k d_k u_k u_k^T over the rows u_k of U (d_ref = 1).
-template <class P>
-CP3_HD void traj_inner_rot(Traj<P>& t, const Rot<P>& r, typename P::D L) {
-  using F = typename P::F;
-  F er, ei;
-  Phase<P>::unit(t.acc, er, ei);
-  C3<F> y2;
-  const C3<F>& a = t.y0;
-  const C3<F>& b = t.y1;
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const int p = (m + 1) % 3, q = (m + 2) % 3;
-    F xr = a.re[p] * b.re[q];
-    xr = xr - a.im[p] * b.im[q];
-    xr = xr - a.re[q] * b.re[p];
-    xr = xr + a.im[q] * b.im[p];
-    F xi = a.re[p] * b.im[q];
-    xi = xi + a.im[p] * b.re[q];
-    xi = xi - a.re[q] * b.im[p];
-    xi = xi - a.im[q] * b.re[p];
-    y2.re[m] = er * xr + ei * xi;
-    y2.im[m] = ei * xr - er * xi;
-  }
-  // ur[k][c], ui[k][c]: component k of R y_c
-  F ur[3][3], ui[3][3];
-  const C3<F>* ys[3] = {&t.y0, &t.y1, &y2};
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    F cr[3], ci[3];
-    rot_apply(r.r, *ys[c], cr, ci);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      ur[k][c] = cr[k];
-      ui[k][c] = ci[k];
-    }
-  }
-  F zr[6], zi[6];
-#pragma unroll
-  for (int n = 0; n < 6; ++n) {
-    const int i = n < 3 ? 0 : (n < 5 ? 1 : 2);
-    const int j = n < 3 ? n : (n < 5 ? n - 2 : 2);
-    zr[n] = ur[2][i] * ur[2][j];
-    zr[n] = zr[n] - ui[2][i] * ui[2][j];
-    zi[n] = ur[2][i] * ui[2][j];
-    zi[n] = zi[n] + ui[2][i] * ur[2][j];
-  }
-  F cr, ci;
-  Phase<P>::evalc(r.oma, L, cr, ci);
-  sym_update(zr, zi, ur[0], ui[0], cr, ci);
-  Phase<P>::evalc(r.omb, L, cr, ci);
-  sym_update(zr, zi, ur[1], ui[1], cr, ci);
-  t.y0.re[0] = zr[0]; t.y0.im[0] = zi[0];
-  t.y0.re[1] = zr[1]; t.y0.im[1] = zi[1];
-  t.y0.re[2] = zr[2]; t.y0.im[2] = zi[2];
-  t.y1.re[0] = zr[3]; t.y1.im[0] = zi[3];
-  t.y1.re[1] = zr[4]; t.y1.im[1] = zi[4];
-  t.y1.re[2] = zr[5]; t.y1.im[2] = zi[5];
-}
-
-// Table entries of either form: rank-one slabs or eigenbasis rotations.
-template <class P>
-struct TabEntryOf {
-  using type = typename std::conditional<P::kRot, Rot<P>, Slab<P>>::type;
-};
-template <class P>
-CP3_HD void shell_middle(Traj<P>& t, const Slab<P>& s, typename P::D L) {
-  traj_middle<P>(t, s, L);
-}
-template <class P>
-CP3_HD void shell_middle(Traj<P>& t, const Rot<P>& r, typename P::D L) {
-  traj_middle_rot<P>(t, r, L);
-}
-template <class P>
-CP3_HD void shell_inner(Traj<P>& t, const Slab<P>& s, typename P::D L) {
-  traj_inner<P>(t, s, L);
-}
-template <class P>
-CP3_HD void shell_inner(Traj<P>& t, const Rot<P>& r, typename P::D L) {
-  traj_inner_rot<P>(t, r, L);
-}
-template <class P>
-CP3_HD void vacuum_slab(const Slab<P>& in, Slab<P>& s) {
-  s = in;
-}
-template <class P>
-CP3_HD void vacuum_slab(const Rot<P>& in, Slab<P>& s) {
-  rot_slab<P>(in, s);
 }
 
 // Applies the atmosphere (vacuum) segment, rotates back to the flavour basis

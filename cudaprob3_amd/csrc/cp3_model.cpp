@@ -45,51 +45,13 @@ void atmos_point(const cp3::SetConst& sc, const cp3::TrajTable& tt, int c,
   for (int k = 0; k < 9; ++k) out9[k] = (double)prob[k];
 }
 
-// The same trajectory in the eigenbasis form the slab-table kernels use: the
-// chain of rotations over all shells of the model, then the vacuum entry.
-template <class P>
-void atmos_point_rot(const cp3::SetConst& sc, const cp3::TrajTable& tt, int c,
-                     double E, double* out9) {
-  using D = typename P::D;
-  using F = typename P::F;
-  const D e = (D)E;
-  const D inv = D(1) / e;
-  const int M = tt.M[(size_t)c];
-  D Vp[9] = {D(1), D(0), D(0), D(0), D(1), D(0), D(0), D(0), D(1)}, V[9];
-  cp3::Traj<P> t;
-  cp3::traj_init<P>(t, M == 0);
-  for (int i = 0; i < M; ++i) {
-    cp3::Rot<P> r;
-    cp3::solve_rot<P>(sc, e, inv, (D)tt.rye[(size_t)i], Vp, V, r);
-    for (int k = 0; k < 9; ++k) Vp[k] = V[k];
-    const D L = (D)tt.seg[(size_t)c * tt.S + (size_t)i];
-    if (i < M - 1)
-      cp3::traj_middle_rot<P>(t, r, L);
-    else
-      cp3::traj_inner_rot<P>(t, r, L);
-  }
-  const D I[9] = {D(1), D(0), D(0), D(0), D(1), D(0), D(0), D(0), D(1)};
-  cp3::Rot<P> rv;
-  cp3::solve_rot<P>(sc, e, inv, D(0), I, V, rv, false);
-  cp3::Slab<P> vac;
-  cp3::rot_slab<P>(rv, vac);
-  F prob[9];
-  cp3::traj_finish<P>(t, vac, (D)tt.Lvac[(size_t)c], sc, prob);
-  for (int k = 0; k < 9; ++k) out9[k] = (double)prob[k];
-}
-
 template <class P>
 void run_atmos(const cp3::SetConst& sc, const cp3::TrajTable& tt,
-               const double* ev, int nE, double* out, bool rot) {
+               const double* ev, int nE, double* out) {
 #pragma omp parallel for schedule(dynamic)
   for (int c = 0; c < tt.nC; ++c)
-    for (int e = 0; e < nE; ++e) {
-      double* o = out + ((size_t)c * nE + e) * 9;
-      if (rot && P::kRot)
-        atmos_point_rot<P>(sc, tt, c, ev[e], o);
-      else
-        atmos_point<P>(sc, tt, c, ev[e], o);
-    }
+    for (int e = 0; e < nE; ++e)
+      atmos_point<P>(sc, tt, c, ev[e], out + ((size_t)c * nE + e) * 9);
 }
 
 template <class P>
@@ -173,17 +135,6 @@ inline void m_cispi_m1(Cnt<T> f, Cnt<T>* wr, Cnt<T>* wi) {
   wr->v = r;
   wi->v = i;
 }
-template <class T>
-inline void m_cispi(Cnt<T> f, Cnt<T>* c, Cnt<T>* s) {
-  if (sizeof(T) == 8)
-    g_ops.sincos64 += 1;
-  else
-    g_ops.sincos32 += 1;
-  T cv, sv;
-  cp3::m_cispi(f.v, &cv, &sv);
-  c->v = cv;
-  s->v = sv;
-}
 inline void m_sincos(Cnt<double> x, Cnt<double>* s, Cnt<double>* c) {
   g_ops.sincos64 += 1;
   s->v = std::sin(x.v);
@@ -201,17 +152,14 @@ namespace cp3 {
 struct CountF64 {
   using D = Cnt<double>;
   using F = Cnt<double>;
-  static constexpr bool kRot = true;
 };
 struct CountF32 {
   using D = Cnt<float>;
   using F = Cnt<float>;
-  static constexpr bool kRot = false;
 };
 struct CountMixed {
   using D = Cnt<double>;
   using F = Cnt<float>;
-  static constexpr bool kRot = true;
 };
 
 }  // namespace cp3
@@ -220,9 +168,8 @@ namespace {
 
 // Counts one call of each kernel stage; out[stage*9 + k] with stages
 // 0 solve_slab, 1 traj_middle, 2 traj_inner, 3 traj_finish (with traj_init),
-// 4 beam_point, 5-7 the slab-table form's solve, middle and innermost shell,
-// and k = f64, f32, sincos64, sincos32, acos64, acos32, sqrt64, sqrt32,
-// rint64.
+// 4 beam_point, and k = f64, f32, sincos64, sincos32, acos64, acos32,
+// sqrt64, sqrt32, rint64.
 template <class P>
 void stage_ops(double* out) {
   using D = typename P::D;
@@ -256,36 +203,13 @@ void stage_ops(double* out) {
   take(3);
   cp3::beam_point<P>(sc, E, D(1.3), D(1300.0), prob);
   take(4);
-  // the slab-table form: the pre-pass step of a shell, and the middle and
-  // innermost shells with its entries (rotations when P::kRot)
-  cp3::traj_init<P>(t, false);
-  if (P::kRot) {
-    const D Vp[9] = {D(1), D(0), D(0), D(0), D(1), D(0), D(0), D(0), D(1)};
-    D V[9];
-    cp3::Rot<P> r;
-    g_ops = OpCount();
-    cp3::solve_rot<P>(sc, E, inv, D(4.0 * 0.494), Vp, V, r);
-    take(5);
-    cp3::traj_middle_rot<P>(t, r, D(700.0));
-    take(6);
-    cp3::traj_inner_rot<P>(t, r, D(1500.0));
-    take(7);
-  } else {
-    g_ops = OpCount();
-    cp3::solve_slab<P>(sc, E, inv, D(4.0 * 0.494), s);
-    take(5);
-    cp3::traj_middle<P>(t, s, D(700.0));
-    take(6);
-    cp3::traj_inner<P>(t, s, D(1500.0));
-    take(7);
-  }
 }
 
 }  // namespace
 
 extern "C" {
 
-// Operation counts of the kernel stages (see stage_ops), 72 doubles.
+// Operation counts of the kernel stages (see stage_ops), 45 doubles.
 void cp3m_stage_ops(int precision, double* out) {
   if (precision == 0)
     stage_ops<cp3::CountF64>(out);
@@ -328,8 +252,7 @@ int cp3m_max_layers(const double* cosv, int nC, const double* radii,
 }
 
 // Atmospheric probabilities [cos][E][9] through the kernel decomposition.
-// precision: 0 = FP64, 1 = FP32, 2 = mixed; precision + 3 selects the
-// eigenbasis (slab-table) form of the same precision.
+// precision: 0 = FP64, 1 = FP32, 2 = mixed.
 int cp3m_atmos(int precision, int nutype, const double* cosv, int nC,
                const double* ev, int nE, const double* radii,
                const double* rhos, const double* yps, int nrows, double th12,
@@ -344,14 +267,12 @@ int cp3m_atmos(int precision, int nutype, const double* cosv, int nC,
     cp3::build_traj_table(tt, em, cosv, nC, h_km * 100000.0);
     const cp3::SetConst sc =
         cp3::make_set(th12, th13, th23, dcp, kNuType, dm12, dm23, nutype);
-    const bool rot = precision >= 3;
-    const int p = precision % 3;
-    if (p == 0)
-      run_atmos<cp3::PolicyF64>(sc, tt, ev, nE, out, rot);
-    else if (p == 1)
-      run_atmos<cp3::PolicyF32>(sc, tt, ev, nE, out, rot);
+    if (precision == 0)
+      run_atmos<cp3::PolicyF64>(sc, tt, ev, nE, out);
+    else if (precision == 1)
+      run_atmos<cp3::PolicyF32>(sc, tt, ev, nE, out);
     else
-      run_atmos<cp3::PolicyMixed>(sc, tt, ev, nE, out, rot);
+      run_atmos<cp3::PolicyMixed>(sc, tt, ev, nE, out);
   } catch (const std::exception& ex) {
     put_err(err, errlen, ex.what());
     return -1;

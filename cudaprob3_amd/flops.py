@@ -20,8 +20,7 @@ import numpy as np
 
 from ._native import MODEL_SO
 
-STAGES = ["solve", "middle", "inner", "finish", "beam", "solve_tab",
-          "middle_tab", "inner_tab"]
+STAGES = ["solve", "middle", "inner", "finish", "beam"]
 KEYS = ["f64", "f32", "sincos64", "sincos32", "acos64", "acos32", "sqrt64",
         "sqrt32", "rint64"]
 PREC = {"fp64": 0, "fp32": 1, "mixed": 2}
@@ -92,11 +91,10 @@ def grid_stage_calls(M: np.ndarray, n_e: int, nc: int, w: int = 1) -> dict:
 
 def table_stage_calls(M: np.ndarray, n_e: int, n_shells: int) -> dict:
     """Stage calls of one atmospheric job in slab-table mode: the pre-pass
-    solves every shell of the Earth model and the vacuum once per energy;
-    middle and innermost shells use the table form's stages."""
-    c = grid_stage_calls(M, n_e, 1, 1)
-    return {"solve_tab": (n_shells + 1) * n_e, "middle_tab": c["middle"],
-            "inner_tab": c["inner"], "finish": c["finish"]}
+    solves every shell of the Earth model and the vacuum once per energy."""
+    calls = grid_stage_calls(M, n_e, 1, 1)
+    calls["solve"] = (n_shells + 1) * n_e
+    return calls
 
 
 def batch_counts(wl, precision: str, nc: int, w: int = 1,

@@ -68,44 +68,21 @@ def build(precision, nc=0, group=0, waves=0, mode="auto"):
     return b, earth, jobs, sets, beam_ev
 
 
-TOL = {"fp64": 1e-10, "mixed": 1e-5}
-_table_first = {}
-
-
-def oracle_atmos(oracle, job, earth):
-    from oracle.loader import Osc
-    t12, t13, t23, d, k, m12, m23 = job.params.for_nutype(job.nutype)
-    ref = oracle.atmos(job.nutype, job.cosv, job.ev, earth.radii, earth.rhos,
-                       earth.yps, Osc(t12, t13, t23, d, m12, m23, k), 22.0,
-                       threads=4)
-    return np.transpose(ref, (2, 0, 1))  # [9][nC][nE]
-
-
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("tuning", [(1, 1, 0, "shared"), (4, 1, -1, "shared"),
                                     (8, 2, 2, "shared"), (2, 4, 0, "shared"),
                                     (4, 4, -1, "shared"), (1, 0, 0, "table"),
                                     (2, 0, -1, "table"), (4, 0, 2, "table"),
                                     (8, 0, -1, "table")])
-def test_batch_equals_single(precision, tuning, oracle):
-    """Shared-solve mode: every job equals the single propagator bitwise.
-    Table mode (eigenbasis chain for FP64 and mixed): every tuning gives the
-    same bits, within the precision's bar of the oracle."""
+def test_batch_equals_single(precision, tuning):
     b, earth, jobs, sets, beam_ev = build(precision, *tuning)
     assert b.tuning()[3] == tuning[3]
     assert b.tuning()[0] == tuning[0]
     b.run()
     b.wait()
     for i, job in enumerate(jobs):
-        got = b.result(i)
-        if tuning[3] == "shared":
-            ref = single_atmos(job, earth, precision)
-            assert np.array_equal(got, ref), (i, job.grid, job.nutype)
-        else:
-            first = _table_first.setdefault((precision, i), got)
-            assert np.array_equal(got, first), (i, job.grid, job.nutype)
-            ref = oracle_atmos(oracle, job, earth)
-            assert np.abs(got - ref).max() < TOL[precision], (i, job.grid)
+        ref = single_atmos(job, earth, precision)
+        assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
     n = len(jobs)
     assert np.array_equal(b.result(n), single_beam(sets[0], 0, beam_ev, precision))
     assert np.array_equal(b.result(n + 1), single_beam(sets[1], 1, beam_ev, precision))
@@ -192,14 +169,12 @@ def test_batch_errors():
 @pytest.mark.parametrize("tuning", [(1, 1, 0, "shared"), (4, 4, -1, "shared"),
                                     (8, 2, 0, "shared"), (1, 0, 0, "table"),
                                     (4, 0, -1, "table"), (8, 0, 0, "table")])
-def test_packed_tails_edge_cases(precision, tuning, oracle):
+def test_packed_tails_edge_cases(precision, tuning):
     """Jobs sharing a grid run their last nE % 64 energies in packed tail
     tiles: a grid with fewer than 64 energies (every lane in tails, several
     tiles, the last one partial), a tail above 32 (not packed), an exact
     multiple of 64 (no tail) and a lone job on a grid (not packed); every
-    result equals the single propagator's bitwise (shared-solve mode), or
-    the first table tuning's bitwise and the oracle within the bar (table
-    mode)."""
+    result equals the single propagator's bitwise."""
     earth = W.earth_59()
     sets = W.draw_param_sets(4, 55)
     grids = {"c": (W.cos_grid(11), W.log_grid(20, 0.5, 50.0)),
@@ -219,13 +194,6 @@ def test_packed_tails_edge_cases(precision, tuning, oracle):
     b.run()
     b.wait()
     for i, job in enumerate(jobs):
-        got = b.result(i)
-        if tuning[3] == "shared":
-            ref = single_atmos(job, earth, precision)
-            assert np.array_equal(got, ref), (i, job.grid, job.nutype)
-        else:
-            first = _table_first.setdefault(("tails", precision, i), got)
-            assert np.array_equal(got, first), (i, job.grid, job.nutype)
-            ref = oracle_atmos(oracle, job, earth)
-            assert np.abs(got - ref).max() < TOL[precision], (i, job.grid)
+        ref = single_atmos(job, earth, precision)
+        assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
     b.close()

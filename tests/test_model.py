@@ -145,30 +145,3 @@ def test_phase_polynomial():
         t = f[n + 1:]
         assert np.all(np.abs(wi[n + 1:] / (2 * np.pi * t) - 1) < 1e-6)
         assert np.all(np.abs(wr[n + 1:] / (-2 * (np.pi * t) ** 2) - 1) < 1e-5)
-
-
-@pytest.mark.parametrize("nutype", [0, 1])
-def test_table_form_fp64_goldens_and_edges(model, nutype):
-    """The eigenbasis (slab-table) form of the chain: FP64 within 1e-10 of the
-    reference goldens, including the degenerate edge cases (dm12 = 0,
-    dm23 = 0), where only an orthonormalised eigenbasis stays exact."""
-    g = golden(f"g1_shells12_{'nubar' if nutype else 'nu'}")
-    e = W.EarthModel(g["radii"], g["rhos"], g["yps"])
-    out = m_atmos(model, 3, nutype, g["cosv"], g["ev"], e, NOM)
-    assert np.abs(out - g["prob"]).max() < 1e-10
-    g = golden("g4_edges")
-    e = W.EarthModel(g["radii"], g["rhos"], g["yps"])
-    for key in ["nominal", "dm12_zero", "dm23_zero", "dcp_zero", "inverted"]:
-        out = m_atmos(model, 3, nutype, g["cosv"], g["ev"], e, g[f"params_{key}"])
-        assert np.abs(out - g[f"prob_{key}_{nutype}"]).max() < 1e-10, key
-
-
-def test_table_form_mixed_59_shells(model, oracle):
-    e = W.earth_59()
-    cz, ev = W.cos_grid(96), W.log_grid(96, 0.5, 50.0)
-    for p in W.draw_param_sets(2, 78):
-        row = [p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
-        for nt in (0, 1):
-            ref = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps,
-                               osc_for(row, nt), 22.0, threads=4)
-            assert np.abs(m_atmos(model, 5, nt, cz, ev, e, row) - ref).max() < 1e-5
