@@ -104,6 +104,7 @@ _SIGS.update({
     "cp3_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "cp3_batch_autotune": (C.c_int, [_vp, C.c_int, _ip, _ip, _ip]),
     "cp3_batch_set_solve_mode": (C.c_int, [_vp, C.c_int]),
+    "cp3_batch_phase_ms": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "cp3_batch_get_tuning": (C.c_int, [_vp, _ip, _ip, _ip, _ip]),
     "cp3_batch_use_graph": (C.c_int, [_vp, C.c_int]),
     "cp3_batch_run": (C.c_int, [_vp]),

@@ -127,6 +127,13 @@ class Batch:
         N.check(self._lib.cp3_batch_last_ms(self._h, C.byref(v)))
         return v.value
 
+    def phase_ms(self) -> dict:
+        """Milliseconds of the last run's phases: slab-table pre-pass,
+        atmospheric propagation, packed tails, beam."""
+        v = (C.c_float * 4)()
+        N.check(self._lib.cp3_batch_phase_ms(self._h, v))
+        return dict(zip(("prepass", "atmos", "tails", "beam"), list(v)))
+
     def stream(self) -> int:
         v = C.c_void_p()
         N.check(self._lib.cp3_batch_get_stream(self._h, C.byref(v)))

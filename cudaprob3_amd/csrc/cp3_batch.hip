@@ -39,6 +39,10 @@ struct cp3_batch {
   int cus = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // recorded at the start of a run's launches and after each phase:
+  // slab-table pre-pass, atmospheric propagation, packed tails, beam (in the
+  // enqueued work itself, so that a replayed graph records them too)
+  hipEvent_t evp[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool ran = false;
   std::vector<std::unique_ptr<AtmosGrid>> agrids;
   std::vector<std::unique_ptr<BeamGrid>> bgrids;
@@ -77,6 +81,8 @@ struct cp3_batch {
     drop_graph();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : evp)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -345,40 +351,49 @@ struct cp3_batch {
     return (int)std::min<int64_t>(units, (int64_t)cus * per_cu);
   }
 
+  void mark(int k) { hip_check(hipEventRecord(evp[k], stream), "hipEventRecord"); }
+
   void enqueue() {
-    if (mode == CP3_SOLVE_TABLE) {
-      const cp3::AtmosJob* aj = (const cp3::AtmosJob*)d_ajobs.p;
-      const int na = (int)ajobs.size();
-      if (s_waves > 0)
-        hip_check(cp3::launch_slab_table(precision, aj, na, s_waves, stream),
-                  "slab table launch");
-      if (a_units > 0)
-        hip_check(cp3::launch_atmos_table(precision, nc, aj, na, a_units, a_grid, stream),
-                  "table-mode atmospheric launch");
-      if (t_units > 0)
-        hip_check(cp3::launch_tail_table(precision, nc, aj,
-                                         (const cp3::TailGroup*)d_tgroups.p,
-                                         (int)tgroups.size(), t_units, stream),
-                  "table-mode tail launch");
-      if (b_units > 0)
-        hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)d_bjobs.p,
-                                         (int)bjobs.size(), b_units, b_grid, stream),
-                  "batched beam launch");
-      return;
-    }
+    const cp3::AtmosJob* aj = (const cp3::AtmosJob*)d_ajobs.p;
+    const int na = (int)ajobs.size();
+    const bool table = mode == CP3_SOLVE_TABLE;
+    mark(0);
+    if (table && s_waves > 0)
+      hip_check(cp3::launch_slab_table(precision, aj, na, s_waves, stream),
+                "slab table launch");
+    mark(1);
     if (a_units > 0)
-      hip_check(cp3::launch_atmos_batch(precision, nc, w, (const cp3::AtmosJob*)d_ajobs.p,
-                                        (int)ajobs.size(), a_units, a_grid, stream),
+      hip_check(table ? cp3::launch_atmos_table(precision, nc, aj, na, a_units, a_grid, stream)
+                      : cp3::launch_atmos_batch(precision, nc, w, aj, na, a_units, a_grid,
+                                                stream),
                 "batched atmospheric launch");
-    if (t_units > 0)
-      hip_check(cp3::launch_atmos_tail(precision, nc, w, (const cp3::AtmosJob*)d_ajobs.p,
-                                       (const cp3::TailGroup*)d_tgroups.p,
-                                       (int)tgroups.size(), t_units, stream),
+    mark(2);
+    if (t_units > 0) {
+      const cp3::TailGroup* tg = (const cp3::TailGroup*)d_tgroups.p;
+      const int ng = (int)tgroups.size();
+      hip_check(table ? cp3::launch_tail_table(precision, nc, aj, tg, ng, t_units, stream)
+                      : cp3::launch_atmos_tail(precision, nc, w, aj, tg, ng, t_units, stream),
                 "packed tail launch");
+    }
+    mark(3);
     if (b_units > 0)
       hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)d_bjobs.p,
                                        (int)bjobs.size(), b_units, b_grid, stream),
                 "batched beam launch");
+    mark(4);
+  }
+
+  // Milliseconds of the last run's phases (about 0 for a phase with no
+  // launch: consecutive markers can read a few microseconds apart either
+  // way, so a negative difference reads as 0).
+  void phase_ms(float out[4]) {
+    if (!ran) throw Error(CP3_ERR_STATE, "the batch was not run");
+    DeviceScope ds(dev);
+    hip_check(hipEventSynchronize(ev1), "hipEventSynchronize");
+    for (int k = 0; k < 4; ++k) {
+      hip_check(hipEventElapsedTime(&out[k], evp[k], evp[k + 1]), "hipEventElapsedTime");
+      if (out[k] < 0.f) out[k] = 0.f;
+    }
   }
 
   void run() {
@@ -518,6 +533,7 @@ int cp3_batch_create(int device, int precision, cp3_batch** out) {
     hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&b->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&b->ev1), "hipEventCreate");
+    for (hipEvent_t& e : b->evp) hip_check(hipEventCreate(&e), "hipEventCreate");
     *out = b.release();
   });
 }
@@ -647,6 +663,14 @@ int cp3_batch_last_ms(cp3_batch* b, float* ms) {
     need(b);
     need_ptr(ms, "ms");
     *ms = b->last_ms();
+  });
+}
+
+int cp3_batch_phase_ms(cp3_batch* b, float* ms) {
+  return guard([&] {
+    need(b);
+    need_ptr(ms, "ms");
+    b->phase_ms(ms);
   });
 }
 

@@ -89,33 +89,58 @@ def grid_stage_calls(M: np.ndarray, n_e: int, nc: int, w: int = 1) -> dict:
     }
 
 
-def table_stage_calls(M: np.ndarray, n_e: int, n_shells: int) -> dict:
-    """Stage calls of one atmospheric job in slab-table mode: the pre-pass
-    solves every shell of the Earth model and the vacuum once per energy."""
-    calls = grid_stage_calls(M, n_e, 1, 1)
-    calls["solve"] = (n_shells + 1) * n_e
-    return calls
+def _zero():
+    return {k: 0.0 for k in KEYS}
+
+
+def batch_phase_counts(wl, precision: str, nc: int, w: int = 1,
+                       mode: str = "shared") -> dict:
+    """Counted work of each phase of a batch launch over an AtmosBatch:
+    "prepass" (slab-table solves), "atmos" (the jobs' own energy tiles) and
+    "tails" (the last nE % 64 energies of jobs sharing a grid, when two or
+    more such tails fit in a wave), following the batch engine's plan."""
+    ops = stage_ops(precision)
+    res = {"prepass": _zero(), "atmos": _zero(), "tails": _zero()}
+    per_grid = {}
+    for j in wl.jobs:
+        per_grid[j.grid] = per_grid.get(j.grid, 0) + 1
+    n_shells = len(wl.earth.radii)
+    cache = {}
+
+    def add(phase, calls):
+        for st, n in calls.items():
+            for k in KEYS:
+                res[phase][k] += n * ops[st][k]
+
+    for j in wl.jobs:
+        n_e = len(j.ev)
+        tail = n_e % 64
+        packed = tail and 64 // tail >= 2 and per_grid[j.grid] >= 2
+        t = tail if packed else 0
+        key = (j.grid, len(j.cosv), n_e)
+        if key not in cache:
+            M = max_layers(j.cosv, wl.earth)
+            if mode == "table":
+                main = grid_stage_calls(M, n_e - t, 1, 1)
+                tl = grid_stage_calls(M, t, 1, 1)
+                main.pop("solve")
+                tl.pop("solve")
+                cache[key] = (main, tl, {"solve": (n_shells + 1) * n_e})
+            else:
+                cache[key] = (grid_stage_calls(M, n_e - t, nc, w),
+                              grid_stage_calls(M, t, nc, w), {})
+        main, tl, pre = cache[key]
+        add("atmos", main)
+        add("tails", tl)
+        add("prepass", pre)
+    return res
 
 
 def batch_counts(wl, precision: str, nc: int, w: int = 1,
                  mode: str = "shared") -> dict:
     """Total counted work of one launch over an AtmosBatch."""
-    ops = stage_ops(precision)
-    tot = {k: 0.0 for k in KEYS}
-    cache = {}
-    for j in wl.jobs:
-        key = (j.grid, len(j.cosv), len(j.ev))
-        if key not in cache:
-            M = max_layers(j.cosv, wl.earth)
-            if mode == "table":
-                cache[key] = table_stage_calls(M, len(j.ev), len(wl.earth.radii))
-            else:
-                cache[key] = grid_stage_calls(M, len(j.ev), nc, w)
-        calls = cache[key]
-        for st, n in calls.items():
-            for k in KEYS:
-                tot[k] += n * ops[st][k]
-    return tot
+    ph = batch_phase_counts(wl, precision, nc, w, mode)
+    return {k: sum(ph[p][k] for p in ph) for k in KEYS}
 
 
 def beam_counts(n_points: int, precision: str) -> dict:

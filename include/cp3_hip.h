@@ -238,6 +238,11 @@ int cp3_batch_run(cp3_batch* b);
 int cp3_batch_wait(cp3_batch* b);
 /* Milliseconds between events around the last run (kernels only). */
 int cp3_batch_last_ms(cp3_batch* b, float* ms);
+/* Milliseconds of the last run's phases, from events recorded between the
+ * launches on the batch's stream: ms[0] slab-table pre-pass, ms[1]
+ * atmospheric propagation, ms[2] packed tail tiles, ms[3] beam (about 0,
+ * within a few microseconds, for a phase that launched nothing). */
+int cp3_batch_phase_ms(cp3_batch* b, float* ms);
 /* Device result of one job ([9][nC][nE] or [9][nE]) and its element size. */
 int cp3_batch_result_device(cp3_batch* b, int job, const void** dptr,
                             int* elem_bytes);

@@ -197,3 +197,26 @@ def test_packed_tails_edge_cases(precision, tuning):
         ref = single_atmos(job, earth, precision)
         assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
     b.close()
+
+
+@pytest.mark.parametrize("mode", ["shared", "table"])
+def test_phase_timings(mode):
+    """Events between the launches split a run into its phases, which lie
+    within the run's time; only table mode has a pre-pass."""
+    b, *_ = build("mixed", mode=mode)
+    for graph in (False, True):
+        b.use_graph(graph)
+        b.run()
+        b.wait()
+        ph = b.phase_ms()
+        assert all(v >= 0 for v in ph.values())
+        # the phases lie within the run (the rest is launch overhead)
+        assert sum(ph.values()) <= b.last_ms() + 0.02
+        if graph:
+            continue
+        assert ph["atmos"] > 0 and ph["beam"] > 0
+        if mode == "table":
+            assert ph["prepass"] > 0
+        else:  # nothing launched: only the events' own spacing
+            assert ph["prepass"] < 0.05
+    b.close()
