@@ -54,21 +54,31 @@ def build_batch(precision, wl, tuning=None, graph=True):
     return b, b.tuning()
 
 
-def time_steps(b, steps, warmup):
+def time_steps(b, steps, warmup, graph):
     """Wall time per step over `steps` runs (after `warmup`), the per-run
-    kernel time between HIP events around the launches, and the mean of
-    each phase (events between the launches on the batch's stream)."""
+    kernel time between HIP events around the launches, and the mean time of
+    each phase: events between the launches on the batch's stream, taken
+    over eager runs of the same launches right after the timed ones (each
+    event then follows its kernel on the stream; a profiler may reorder the
+    markers inside a replayed graph)."""
+    b.use_graph(graph)
     for _ in range(warmup):
         b.run()
     b.wait()
-    kernel_ms, phases = [], []
+    kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
         b.run()
         b.wait()
         kernel_ms.append(b.last_ms())
-        phases.append(b.phase_ms())
     t1 = time.perf_counter()
+    b.use_graph(False)
+    phases = []
+    for _ in range(max(3, steps // 2)):
+        b.run()
+        b.wait()
+        phases.append(b.phase_ms())
+    b.use_graph(graph)
     mean_ph = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
     return (t1 - t0) * 1e3 / steps, np.array(kernel_ms), mean_ph
 
@@ -181,7 +191,7 @@ def main():
     if args.tune != "auto":
         fixed = tuple(args.tune.split(","))
     mixed, tune_m = build_batch("mixed", wl, tuning=fixed, graph=graph)
-    ms_step, kms, phases = time_steps(mixed, args.steps, args.warmup)
+    ms_step, kms, phases = time_steps(mixed, args.steps, args.warmup, graph)
     value = evals / (ms_step / 1e3)
 
     # roofline of the dominant kernel (the atmospheric propagation phase):
@@ -239,8 +249,8 @@ def main():
             "note": "FP64-equivalent rate (F64 + F32*78.6/157.3) of the "
                     "counted FLOPs of the exact kernel decomposition over the "
                     "atmospheric propagation kernel's own launch time "
-                    "(events between the launches); transcendentals counted "
-                    "separately",
+                    "(HIP events between the launches of eager runs of the "
+                    "timed batch); transcendentals counted separately",
             "flops_f64": counts["f64"], "flops_f32": counts["f32"],
             "sincos_f64": counts["sincos64"], "sincos_f32": counts["sincos32"],
             "batch_achieved": batch_rate / 1e12,
@@ -252,7 +262,7 @@ def main():
 
     if not args.no_fp64:
         fp64, tune_d = build_batch("fp64", wl, tuning=fixed, graph=graph)
-        ms64, kms64, ph64 = time_steps(fp64, max(3, args.steps // 2), 1)
+        ms64, kms64, ph64 = time_steps(fp64, max(3, args.steps // 2), 1, graph)
         v64 = evals / (ms64 / 1e3)
         c64 = FL.batch_phase_counts(wl, "fp64", tune_d[0], tune_d[1], tune_d[3])["atmos"]
         t64 = ph64["atmos"] / 1e3
