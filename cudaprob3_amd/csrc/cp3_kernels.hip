@@ -490,11 +490,15 @@ __global__ void __launch_bounds__(kWave, (MinWavesTable<P, NC>::value)) atmos_ta
   }
 }
 
+// Table-mode packed tails, with the same XCD-local walk of the unit list.
 template <class P, int NC>
 __global__ void __launch_bounds__(kWave, (MinWavesTable<P, NC>::value)) tail_table_kernel(
     const AtmosJob* __restrict__ jobs, const TailGroup* __restrict__ groups, int ngroups,
     int64_t nunits) {
-  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+  const int64_t per = (nunits + 7) / 8;
+  for (int64_t b = blockIdx.x; b < per * 8; b += gridDim.x) {
+    const int64_t u = (b % 8) * per + b / 8;
+    if (u >= nunits) continue;
     int lo = 0, hi = ngroups - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -692,10 +696,11 @@ template <class P>
 static hipError_t tail_table_p(int nc, const AtmosJob* jobs, const TailGroup* groups,
                                int ngroups, int64_t nunits, hipStream_t st) {
   if (nunits <= 0 || ngroups <= 0) return hipSuccess;
-  if (nunits > 0x7fffffff) return hipErrorInvalidConfiguration;
+  const int64_t grid = (nunits + 7) / 8 * 8;
+  if (grid > 0x7fffffff) return hipErrorInvalidConfiguration;
 #define X(NC_)                                                                   \
   if (nc == NC_) {                                                               \
-    tail_table_kernel<P, NC_><<<dim3((unsigned)nunits), dim3(kWave), 0, st>>>(    \
+    tail_table_kernel<P, NC_><<<dim3((unsigned)grid), dim3(kWave), 0, st>>>(      \
         jobs, groups, ngroups, nunits);                                          \
     return hipGetLastError();                                                    \
   }
