@@ -77,10 +77,22 @@ void auto_tuning(int precision, const cp3::GridDev& g, int* nc, int* w) {
   if (cp3::atmos_units(g, 1, *w) * *w < 4096) *w = 1;
 }
 
+// Slab-table mode: the largest cosine block up to 4 that still leaves 2048
+// work units (8 per CU), else 1.
+int table_nc(const cp3::GridDev& g) {
+  for (int c = 4; c > 1; c /= 2)
+    if (cp3::cosine_blocks(g, c, 1) * cp3::energy_tiles(g) >= 2048) return c;
+  return 1;
+}
+
 }  // namespace
 
 struct cp3_atmos {
   int dev = 0, nC = 0, nE = 0, precision = CP3_FP64, nc_force = 0, w_force = 0;
+  int solve_mode = CP3_SOLVE_AUTO;
+  // slab-table mode: the table and the job descriptor on the device
+  DevBuf d_slabs, d_job;
+  PinnedBuf h_job;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool launched = false;
@@ -150,14 +162,37 @@ struct cp3_atmos {
     job.out = d_out.p;
     job.unit_begin = 0;
     job.etiles = cp3::energy_tiles(job.g);
-    int nc = 1, w = 1;
-    auto_tuning(precision, job.g, &nc, &w);
-    if (nc_force) nc = nc_force;
-    if (w_force) w = w_force;
     results_on_host = false;
-    hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
-    hip_check(cp3::launch_atmos_single(precision, nc, w, job, stream),
-              "atmospheric kernel launch");
+    if (solve_mode == CP3_SOLVE_SHARED) {
+      int nc = 1, w = 1;
+      auto_tuning(precision, job.g, &nc, &w);
+      if (nc_force) nc = nc_force;
+      if (w_force) w = w_force;
+      hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+      hip_check(cp3::launch_atmos_single(precision, nc, w, job, stream),
+                "atmospheric kernel launch");
+    } else {
+      // the grid's slab table (a pre-pass) read by the propagation kernel
+      const int nc = nc_force ? nc_force : table_nc(job.g);
+      d_slabs.alloc((size_t)cp3::slab_waves(job.g) * cp3::kWave * cp3::slab_entry_bytes(precision),
+                    dev);
+      job.slabs = d_slabs.p;
+      // the pinned staging may still feed the previous launch's copy
+      hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      h_job.alloc(sizeof(job));
+      std::memcpy(h_job.p, &job, sizeof(job));
+      d_job.alloc(sizeof(job), dev);
+      hip_check(hipMemcpyAsync(d_job.p, h_job.p, sizeof(job), hipMemcpyHostToDevice, stream),
+                "job upload");
+      const cp3::AtmosJob* dj = (const cp3::AtmosJob*)d_job.p;
+      const int64_t units = cp3::cosine_blocks(job.g, nc, 1) * job.etiles;
+      hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+      hip_check(cp3::launch_slab_table(precision, dj, 1, cp3::slab_waves(job.g), stream),
+                "slab table launch");
+      hip_check(cp3::launch_atmos_table(precision, nc, dj, 1, units,
+                                        (int)((units + 7) / 8 * 8), stream),
+                "atmospheric kernel launch");
+    }
     hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
     launched = true;
   }
@@ -503,6 +538,15 @@ int cp3_atmos_set_tuning(cp3_atmos* p, int nc, int w) {
       throw Error(CP3_ERR_INVALID, "group waves must be 0, 1, 2 or 4");
     p->nc_force = nc;
     p->w_force = w;
+  });
+}
+
+int cp3_atmos_set_solve_mode(cp3_atmos* p, int mode) {
+  return guard([&] {
+    need(p);
+    if (mode != CP3_SOLVE_AUTO && mode != CP3_SOLVE_SHARED && mode != CP3_SOLVE_TABLE)
+      throw Error(CP3_ERR_INVALID, "solve mode must be CP3_SOLVE_AUTO, _SHARED or _TABLE");
+    p->solve_mode = mode;
   });
 }
 

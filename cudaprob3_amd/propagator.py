@@ -114,6 +114,14 @@ class AtmosPropagator:
         N.check(self._lib.cp3_atmos_set_tuning(self._h, int(cosine_block),
                                                int(group_waves)))
 
+    def setSolveMode(self, mode: str = "auto"):
+        """Where the eigen-solutions come from: "table" (solved once per
+        shell and energy into device memory by a pre-pass), "shared" (solved
+        in the propagation kernel, shared through LDS) or "auto" (the
+        table); results do not depend on it."""
+        N.check(self._lib.cp3_atmos_set_solve_mode(self._h,
+                                                   N.SOLVE_MODES[mode]))
+
     # calculation and results
     def calculateProbabilities(self, nutype=Neutrino):
         N.check(self._lib.cp3_atmos_calculate(self._h, int(nutype)))

@@ -124,9 +124,14 @@ int cp3_atmos_get_stream(cp3_atmos* p, void** stream);
 /* Milliseconds of the last kernel, measured with HIP events around it. */
 int cp3_atmos_last_kernel_ms(cp3_atmos* p, float* ms);
 /* Kernel tuning: trajectories per lane (cosine block 1, 2, 4 or 8) and waves
- * per workgroup sharing each eigen-solve through LDS (1, 2 or 4); 0 =
- * automatic.  Results do not depend on it (bitwise). */
+ * per workgroup sharing each eigen-solve through LDS (1, 2 or 4; shared
+ * solve mode only); 0 = automatic.  Results do not depend on it (bitwise). */
 int cp3_atmos_set_tuning(cp3_atmos* p, int cosine_block, int group_waves);
+/* Where the per-(energy, shell) eigen-solutions come from, as for the batch
+ * (CP3_SOLVE_SHARED, CP3_SOLVE_TABLE; CP3_SOLVE_AUTO = the table, which for
+ * one grid takes (shells + 1) x energies x 40 B in mixed precision).  No
+ * counterpart in the reference; results do not depend on it (bitwise). */
+int cp3_atmos_set_solve_mode(cp3_atmos* p, int mode);
 
 /* ---------------------------------------------------------------------------
  * Beam propagator (one energy list, one parameter set per calculate).

@@ -19,9 +19,10 @@ TOL_FP32 = 2e-3
 
 
 def run_atmos(cz, ev, radii, rhos, yps, params, nutype, precision="fp64",
-              h_km=22.0, nc=0, w=0):
+              h_km=22.0, nc=0, w=0, mode="auto"):
     from cudaprob3_amd import AtmosPropagator
     with AtmosPropagator(len(cz), len(ev), precision=precision) as p:
+        p.setSolveMode(mode)
         if nc or w:
             p.setTuning(nc, w)
         p.setEnergyList(ev)
@@ -117,17 +118,24 @@ def test_beam_fp64_vs_golden():
             assert unitarity(out) < 1e-12
 
 
-@pytest.mark.parametrize("nc", [1, 2, 4, 8])
-@pytest.mark.parametrize("w", [1, 2, 4])
-def test_tunings_identical(nc, w):
-    """The cosine block and the waves sharing a solve change only which lanes
-    share an eigen-solve, never a point's arithmetic: bitwise identical.
-    37 cosines leave ragged last blocks (and idle waves) for every tuning."""
+TUNINGS = ([("shared", nc, w) for w in (1, 2, 4) for nc in (1, 2, 4, 8)]
+           + [("table", nc, 1) for nc in (1, 2, 4, 8)] + [("auto", 0, 0)])
+
+
+@pytest.mark.parametrize("mode,nc,w", TUNINGS)
+def test_tunings_identical(mode, nc, w):
+    """The solve mode, the cosine block and the waves sharing a solve change
+    only where a point's eigen-solutions are made and which lanes share them,
+    never a point's arithmetic: bitwise identical to the shared solve with
+    one cosine per lane.  37 cosines leave ragged last blocks (and idle
+    waves) for every tuning; 70 energies a partial energy tile."""
     e = W.earth_12()
     cz, ev = W.cos_grid(37), W.log_grid(70, 0.1, 100.0)
     for prec in ("fp64", "fp32", "mixed"):
-        b = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=1, w=1)
-        o = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=nc, w=w)
+        b = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=1, w=1,
+                      mode="shared")
+        o = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, prec, nc=nc, w=w,
+                      mode=mode)
         assert np.array_equal(b, o), prec
 
 
