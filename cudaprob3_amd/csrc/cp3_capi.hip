@@ -92,7 +92,6 @@ struct cp3_atmos {
   int solve_mode = CP3_SOLVE_AUTO;
   // slab-table mode: the table and the job descriptor on the device
   DevBuf d_slabs, d_job;
-  PinnedBuf h_job;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool launched = false;
@@ -177,14 +176,9 @@ struct cp3_atmos {
       d_slabs.alloc((size_t)cp3::slab_waves(job.g) * cp3::kWave * cp3::slab_entry_bytes(precision),
                     dev);
       job.slabs = d_slabs.p;
-      // the pinned staging may still feed the previous launch's copy
-      hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-      h_job.alloc(sizeof(job));
-      std::memcpy(h_job.p, &job, sizeof(job));
       d_job.alloc(sizeof(job), dev);
-      hip_check(hipMemcpyAsync(d_job.p, h_job.p, sizeof(job), hipMemcpyHostToDevice, stream),
-                "job upload");
-      const cp3::AtmosJob* dj = (const cp3::AtmosJob*)d_job.p;
+      cp3::AtmosJob* dj = (cp3::AtmosJob*)d_job.p;
+      hip_check(cp3::launch_store_job(job, dj, stream), "job descriptor store");
       const int64_t units = cp3::cosine_blocks(job.g, nc, 1) * job.etiles;
       hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
       hip_check(cp3::launch_slab_table(precision, dj, 1, cp3::slab_waves(job.g), stream),

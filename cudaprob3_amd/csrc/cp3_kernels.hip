@@ -384,6 +384,16 @@ __global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_tai
 // table among every cosine of the grid).  Table of a job: Slab<P> rows
 // [g.S + 1][energy_tiles(g) * 64], row g.S the vacuum.
 
+// Writes one job descriptor, passed as a kernel argument, to device memory in
+// stream order (no host staging buffer to keep alive until the copy runs).
+__global__ void __launch_bounds__(kWave) store_job_kernel(AtmosJob job, AtmosJob* dst) {
+  static_assert(sizeof(AtmosJob) % 4 == 0, "AtmosJob is copied as 32-bit words");
+  constexpr int kWords = (int)(sizeof(AtmosJob) / 4);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&job);
+  uint32_t* out = reinterpret_cast<uint32_t*>(dst);
+  for (int i = (int)threadIdx.x; i < kWords; i += kWave) out[i] = src[i];
+}
+
 // Pre-pass: one wave per (job, shell or vacuum, 64-energy tile), on a 2-D
 // grid (y: job, x: the job's shell x tile) so that a wave finds its job
 // without a search; waves past a smaller job's count exit at once.
@@ -650,6 +660,11 @@ size_t slab_entry_bytes(int precision) {
     case kMixed: return sizeof(Slab<PolicyMixed>);
     default: return 0;
   }
+}
+
+hipError_t launch_store_job(const AtmosJob& job, AtmosJob* dst, hipStream_t st) {
+  store_job_kernel<<<dim3(1), dim3(kWave), 0, st>>>(job, dst);
+  return hipGetLastError();
 }
 
 hipError_t launch_slab_table(int precision, const AtmosJob* jobs, int njobs,
