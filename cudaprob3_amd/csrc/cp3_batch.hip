@@ -174,9 +174,8 @@ struct cp3_batch {
       const size_t a = (size_t)slot[j];
       const cp3::GridDev g = agrids[(size_t)jobs[j].grid]->dev();
       slab_off[a] = off;
-      const int64_t waves = cp3::slab_waves(g);
-      off = align_up(off + (size_t)waves * cp3::kWave * entry);
-      wv = std::max(wv, waves);
+      off = align_up(off + (size_t)cp3::slab_entries(g) * entry);
+      wv = std::max(wv, cp3::slab_waves(g));
     }
     slab_total = off;
     s_waves = wv;

@@ -173,8 +173,7 @@ struct cp3_atmos {
     } else {
       // the grid's slab table (a pre-pass) read by the propagation kernel
       const int nc = nc_force ? nc_force : table_nc(job.g);
-      d_slabs.alloc((size_t)cp3::slab_waves(job.g) * cp3::kWave * cp3::slab_entry_bytes(precision),
-                    dev);
+      d_slabs.alloc((size_t)cp3::slab_entries(job.g) * cp3::slab_entry_bytes(precision), dev);
       job.slabs = d_slabs.p;
       d_job.alloc(sizeof(job), dev);
       cp3::AtmosJob* dj = (cp3::AtmosJob*)d_job.p;

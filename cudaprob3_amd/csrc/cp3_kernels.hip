@@ -394,26 +394,34 @@ __global__ void __launch_bounds__(kWave) store_job_kernel(AtmosJob job, AtmosJob
   for (int i = (int)threadIdx.x; i < kWords; i += kWave) out[i] = src[i];
 }
 
-// Pre-pass: one wave per (job, shell or vacuum, 64-energy tile), on a 2-D
-// grid (y: job, x: the job's shell x tile) so that a wave finds its job
-// without a search; waves past a smaller job's count exit at once.
+// Pre-pass: the (S + 1) x nE used entries of each job's table, 64
+// consecutive (row, energy) pairs per wave, row-major, so that no lane solves
+// a padding column; on a 2-D grid (y: job, x: the job's waves) so that a wave
+// finds its job without a search; waves past a smaller job's count exit at
+// once.
 template <class P>
 __global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __restrict__ jobs) {
   using D = typename P::D;
   const int lane = (int)threadIdx.x;
   const AtmosJob& job = jobs[blockIdx.y];
   const GridDev& g = job.g;
-  const int64_t u = blockIdx.x;
-  if (u >= slab_waves(g)) return;
-  const int et = energy_tiles(g);
-  const int i = (int)(u / et);
-  const int e = (int)(u - (int64_t)i * et) * kWave + lane;
-  const D E = (D)g.ev[e < g.nE ? e : g.nE - 1];
+  if ((int64_t)blockIdx.x >= slab_waves(g)) return;
+  // the wave's first pair (row i0, energy e0), wave-uniform
+  const int64_t first = (int64_t)blockIdx.x * kWave;
+  const int i0 = (int)(first / g.nE);
+  int i = i0;
+  int e = (int)(first - (int64_t)i0 * g.nE) + lane;
+  while (e >= g.nE) {  // at most ceil(64 / nE) steps
+    e -= g.nE;
+    ++i;
+  }
+  if (i > g.S) return;  // past the last row
+  const D E = (D)g.ev[e];
   // g.S >= 1 (an Earth model has rows), so row 0 of rye exists
-  const D rye = (D)uload(g.rye, i < g.S ? i : 0);
+  const D rye = (D)g.rye[i < g.S ? i : 0];
   Slab<P> sl;
   solve_slab<P>(job.sc, E, D(1) / E, i < g.S ? rye : D(0), sl);
-  ((Slab<P>*)job.slabs)[(int64_t)i * et * kWave + e] = sl;
+  ((Slab<P>*)job.slabs)[(int64_t)i * energy_tiles(g) * kWave + e] = sl;
 }
 
 // The NC trajectories of cosine block sb (NC cosines) for one energy per

@@ -75,9 +75,13 @@ CP3_HD int64_t cosine_blocks(const GridDev& g, int nc, int w) {
   return (g.nC + nc * w - 1) / (nc * w);
 }
 CP3_HD int32_t energy_tiles(const GridDev& g) { return (g.nE + kWave - 1) / kWave; }
-// Pre-pass waves of one job's slab table: (shell or vacuum) x energy tile.
+// A job's slab table: rows (shells, then the vacuum) of energy_tiles * 64
+// entries; the pre-pass solves its (S + 1) x nE used entries in waves of 64.
+CP3_HD int64_t slab_entries(const GridDev& g) {
+  return (int64_t)(g.S + 1) * energy_tiles(g) * kWave;
+}
 CP3_HD int64_t slab_waves(const GridDev& g) {
-  return (int64_t)(g.S + 1) * energy_tiles(g);
+  return ((int64_t)(g.S + 1) * g.nE + kWave - 1) / kWave;
 }
 
 // Work units of one job over all its energy tiles: (cosine block) x (tile).
