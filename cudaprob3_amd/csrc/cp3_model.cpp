@@ -152,14 +152,17 @@ namespace cp3 {
 struct CountF64 {
   using D = Cnt<double>;
   using F = Cnt<double>;
+  static constexpr bool kRateAcc = PolicyF64::kRateAcc;
 };
 struct CountF32 {
   using D = Cnt<float>;
   using F = Cnt<float>;
+  static constexpr bool kRateAcc = PolicyF32::kRateAcc;
 };
 struct CountMixed {
   using D = Cnt<double>;
   using F = Cnt<float>;
+  static constexpr bool kRateAcc = PolicyMixed::kRateAcc;
 };
 
 }  // namespace cp3

@@ -107,10 +107,11 @@ def test_stage_counts():
     mixed = FL.stage_ops("mixed")
     # a middle layer: two rank-one updates of two complex 3-vectors each
     # (6 + 4 + 6 multiply-adds = 28 FLOPs per update) in F, and per phase
-    # q = om L, q - rint(q) and the determinant sum in D
+    # q = om L and q - rint(q) in D; the determinant phase: FP64 adds each
+    # reduced phase (2 per layer), mixed advances by (oma + omb) L (3)
     assert mixed["middle"]["f32"] == 4 * 28
-    assert mixed["middle"]["f64"] == 2 * 3
-    assert f64["middle"]["f64"] == mixed["middle"]["f32"] + mixed["middle"]["f64"]
+    assert mixed["middle"]["f64"] == 2 * 2 + 3
+    assert f64["middle"]["f64"] == 4 * 28 + 2 * 3
     assert f64["middle"]["sincos64"] == 2 and mixed["middle"]["sincos32"] == 2
     assert f64["solve"]["acos64"] == 1 and mixed["solve"]["acos64"] == 1
     assert mixed["solve"]["f32"] == 0  # the eigen-solve stays FP64
