@@ -13,9 +13,12 @@ pytestmark = pytest.mark.gpu
 
 
 def single_atmos(job, earth, precision):
+    """The single propagator solving inside its kernel (shared mode), an
+    independent path from the batch's slab table."""
     from cudaprob3_amd import AtmosPropagator
     t12, t13, t23, d, k, m12, m23 = job.params.for_nutype(job.nutype)
     with AtmosPropagator(len(job.cosv), len(job.ev), precision=precision) as p:
+        p.setSolveMode("shared")
         p.setEnergyList(job.ev)
         p.setCosineList(job.cosv)
         p.setMNSMatrix(t12, t13, t23, d, k)
@@ -193,6 +196,33 @@ def test_packed_tails_edge_cases(precision, tuning):
     assert b.tuning()[3] == tuning[3]
     b.run()
     b.wait()
+    for i, job in enumerate(jobs):
+        ref = single_atmos(job, earth, precision)
+        assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
+    b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode", ["shared", "table"])
+def test_tiny_grids(precision, mode):
+    """Grids of 1, 3 and 5 energies and 1 or 2 cosines: a pre-pass wave
+    spans many table rows (every lane walks its own row), every lane of a
+    work unit but a few is masked; results equal the single propagator's
+    bitwise."""
+    earth = W.earth_59()
+    sets = W.draw_param_sets(2, 77)
+    grids = {"g1": (W.cos_grid(1), W.log_grid(1, 0.5, 50.0)),
+             "g3": (W.cos_grid(2), W.log_grid(3, 0.5, 50.0)),
+             "g5": (np.array([-0.99, -0.05]), W.log_grid(5, 0.1, 100.0))}
+    plan = [(g, s, nt) for g in grids for s in sets for nt in (0, 1)]
+    b = Batch(precision)
+    b.set_solve_mode(mode)
+    gid = {k: b.add_atmos_grid(c, e, earth, 22.0) for k, (c, e) in grids.items()}
+    jobs = [W.AtmosJob(p, grids[g][0], grids[g][1], nt, grid=g) for g, p, nt in plan]
+    b.set_jobs([make_job(ATMOS, gid[j.grid], j.params, j.nutype) for j in jobs])
+    b.run()
+    b.wait()
+    assert b.tuning()[3] == mode
     for i, job in enumerate(jobs):
         ref = single_atmos(job, earth, precision)
         assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
