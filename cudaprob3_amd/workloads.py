@@ -101,6 +101,48 @@ def prem4() -> EarthModel:
                       np.full(5, 0.5))
 
 
+@dataclass
+class PolyEarthModel:
+    """Rows ascending by radius with the 0-km row first; the shell whose outer
+    radius is row i has density a[i] R^2 + b[i] R + c[i] g/cm^3 at radius R km,
+    averaged along each trajectory segment (setDensity(radii, a, b, c, yps),
+    atmoscpupropagator.hpp:181-245; physics.hpp:653-739)."""
+
+    radii: np.ndarray
+    a: np.ndarray
+    b: np.ndarray
+    c: np.ndarray
+    yps: np.ndarray
+
+    def write(self, path: str):
+        """The 5-column density-file format (radius, a, b, c, Ye)."""
+        with open(path, "w") as f:
+            for row in zip(self.radii, self.a, self.b, self.c, self.yps):
+                f.write(" ".join(repr(float(x)) for x in row) + "\n")
+
+    def rho(self, shell: int, r_km):
+        """Density of row `shell` at radius r_km."""
+        return self.a[shell] * r_km ** 2 + self.b[shell] * r_km + self.c[shell]
+
+
+def prem4_poly() -> PolyEarthModel:
+    """Four shells at the PREM radii with quadratic density profiles in
+    x = R / 6371 km: inner core 13.0885 - 8.8381 x^2, outer core
+    12.5815 - 1.2638 x - 3.6426 x^2, lower mantle 7.9565 - 6.4761 x +
+    5.5283 x^2, upper mantle and crust 5.3197 - 1.4836 x; Ye 0.466 in the
+    core, 0.494 outside.  The 0-km row repeats the inner core's row."""
+    R = R_EARTH_KM
+    shells = [(-8.8381, 0.0, 13.0885), (-3.6426, -1.2638, 12.5815),
+              (5.5283, -6.4761, 7.9565), (0.0, -1.4836, 5.3197)]
+    rows = [shells[0]] + shells
+    a = np.array([s[0] / R ** 2 for s in rows])
+    b = np.array([s[1] / R for s in rows])
+    c = np.array([s[2] for s in rows])
+    yps = np.array([0.466, 0.466, 0.466, 0.494, 0.494])
+    return PolyEarthModel(np.array([0.0, 1220.0, 3480.0, 5701.0, 6371.0]),
+                          a, b, c, yps)
+
+
 def seeded_earth(n_shells: int, seed: int) -> EarthModel:
     """n_shells shells: n_shells-1 outer radii sorted uniform in (0, 6371) km
     plus 6371 km; densities linear in shell index from 13.0 (innermost) to 2.6

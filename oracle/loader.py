@@ -45,6 +45,13 @@ def _atmos_file_argtypes(outp):
             C.c_double, C.c_double, C.c_int, outp, C.c_char_p, C.c_int]
 
 
+def _atmos_poly_argtypes(outp):
+    return [C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp,
+            C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+            C.c_double, C.c_double, C.c_double, C.c_int, outp, C.c_char_p,
+            C.c_int]
+
+
 def _beam_argtypes(outp):
     return [C.c_int, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
             C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
@@ -87,6 +94,9 @@ class PropLib:
         self._atmos_file = getattr(L, prefix + "atmos_file_f64")
         self._atmos_file.argtypes = _atmos_file_argtypes(_dp)
         self._atmos_file.restype = C.c_int
+        self._atmos_poly = getattr(L, prefix + "atmos_poly_f64")
+        self._atmos_poly.argtypes = _atmos_poly_argtypes(_dp)
+        self._atmos_poly.restype = C.c_int
         self._beam = getattr(L, prefix + "beam_f64")
         self._beam.argtypes = _beam_argtypes(_dp)
         self._beam.restype = C.c_int
@@ -120,6 +130,22 @@ class PropLib:
         rc = fn(int(nutype), _ptr(cosv), len(cosv), _ptr(ev), len(ev),
                 _ptr(radii), _ptr(rhos), _ptr(yps), len(radii), *osc.args(),
                 float(h_km), int(threads), _ptr(out, op), err, 512)
+        self._check(rc, err)
+        return out
+
+    def atmos_poly(self, nutype: int, cosv, ev, radii, a, b, c, yps, osc: Osc,
+                   h_km: float, threads: int = 1):
+        """setDensity(radii, a, b, c, yps): row densities a R^2 + b R + c
+        (R in km) averaged per trajectory segment."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float64)
+                for x in (cosv, ev, radii, a, b, c, yps)]
+        cosv, ev, radii, a, b, c, yps = arrs
+        err = C.create_string_buffer(512)
+        out = np.zeros((len(cosv), len(ev), 9), dtype=np.float64)
+        rc = self._atmos_poly(int(nutype), _ptr(cosv), len(cosv), _ptr(ev),
+                              len(ev), _ptr(radii), _ptr(a), _ptr(b), _ptr(c),
+                              _ptr(yps), len(radii), *osc.args(), float(h_km),
+                              int(threads), _ptr(out), err, 512)
         self._check(rc, err)
         return out
 
@@ -158,6 +184,9 @@ class Oracle(PropLib):
         super().__init__(path, "orc_")
         L = self.lib
         L.orc_pmns.argtypes = [C.c_double] * 4 + [C.c_int, _dp]
+        L.orc_poly_rye.argtypes = ([_dp, C.c_int] + [_dp] * 5 +
+                                   [C.c_int, C.c_double, _dp, C.c_char_p, C.c_int])
+        L.orc_poly_rye.restype = C.c_int
         L.orc_pmns.restype = None
         L.orc_matter_eigs.argtypes = ([C.c_double] * 4 + [C.c_int] +
                                       [C.c_double] * 4 + [C.c_int, _dp, _ip])
@@ -168,6 +197,20 @@ class Oracle(PropLib):
         L.orc_segments.argtypes = [C.c_double, _dp, _dp, _dp, C.c_int,
                                    C.c_double, _dp, _dp, C.c_int]
         L.orc_segments.restype = C.c_int
+
+    def poly_rye(self, cosv, radii, a, b, c, yps, h_km):
+        """rho*Ye of each (cosine, layer 1..rows) of a polynomial model as
+        the propagation uses it ([nC][rows], 0 past the innermost layer)."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float64)
+                for x in (cosv, radii, a, b, c, yps)]
+        cosv, radii, a, b, c, yps = arrs
+        out = np.zeros((len(cosv), len(radii)))
+        err = C.create_string_buffer(512)
+        rc = self.lib.orc_poly_rye(_ptr(cosv), len(cosv), _ptr(radii), _ptr(a),
+                                   _ptr(b), _ptr(c), _ptr(yps), len(radii),
+                                   float(h_km), _ptr(out), err, 512)
+        self._check(rc, err)
+        return out
 
     def pmns(self, th12, th13, th23, dcp, knutype=1):
         out = np.zeros(18)

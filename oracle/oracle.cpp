@@ -15,7 +15,11 @@
 //     model is passed without its 0-km row, instead of reading past the end;
 //     the value is unused on that branch (physics.hpp:780-786);
 //   * unsupported density files raise an error instead of std::terminate
-//     (atmoscpupropagator.hpp:338-342).
+//     (atmoscpupropagator.hpp:338-342);
+//   * polynomial models need b and c lists as long as the radii (the
+//     reference checks only a's size, atmoscpupropagator.hpp:190-200, and
+//     would read past a shorter list), and the cos = -1 branch of the
+//     polynomial average reads radius 0 past a model without its 0-km row.
 //
 // Pins (tests/): golden vectors made by the compiled reference
 // (tests/golden/), the reference library itself where it is built
@@ -24,7 +28,9 @@
 // and invariants (unitarity, nu-bar <-> (delta, A) -> (-delta, -A), T-symmetry
 // at delta = 0).
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -324,10 +330,60 @@ double layer_rho_ye(const double* rhos, const double* yps, int layer,
 }
 
 // An Earth model as AtmosCpuPropagator<double> holds it after setDensity
-// (atmoscpupropagator.hpp:116-170): rows descending, coslimit per row.
+// (atmoscpupropagator.hpp:116-170): rows descending, coslimit per row.  With
+// `poly` (setDensity(radii, a, b, c, yps), :181-245) the density of row i is
+// rho(R) = as[i] R^2 + bs[i] R + cs[i], R in km, averaged per trajectory.
 struct Earth {
   std::vector<double> radii, rhos, yps, coslimit;
+  std::vector<double> as, bs, cs;
+  bool poly = false;
 };
+
+// physics.hpp:653-739 (getDensityOfLayerPoly): rho*Ye averaged over the
+// layer's segment of dist_cm, the segment parametrised from the shell's outer
+// boundary inward by t (km) with R(t)^2 = t^2 - 2 d t + R_i^2, d the half chord
+// at R_i.  The integrals of R^2, R and 1 over t in [0, t2] are the closed
+// forms of the reference, in its operation order.
+double layer_rho_ye_poly(const Earth& e, double cz, int layer, int max_layer,
+                         double dist_cm) {
+  if (cz >= 0) return 0.0;
+  if (layer == 0) return 0.0;
+  const int i = (layer <= max_layer) ? layer - 1 : 2 * max_layer - layer - 1;
+  const size_t n = e.radii.size();
+  const double Ri = e.radii[(size_t)i];
+  const double Rn = ((size_t)i + 1 < n) ? e.radii[(size_t)i + 1] : 0.0;
+  const double A = e.as[(size_t)i], B = e.bs[(size_t)i], Cc = e.cs[(size_t)i];
+  const double rmin2 = std::pow(kREarthKm, 2) * (1 - std::pow(cz, 2));
+  const double rmax2 = std::pow(Ri, 2);
+  const double half = std::sqrt(rmax2 - rmin2);
+  if (rmin2 == 0) {
+    // straight through the centre: the radial average over [Rn, Ri]
+    const double hi = A * std::pow(Ri, 3) / 3. + B * std::pow(Ri, 2) / 2. + Cc * Ri;
+    const double lo = A * std::pow(Rn, 3) / 3. + B * std::pow(Rn, 2) / 2. + Cc * Rn;
+    double rho = hi - lo;
+    rho /= (Ri - Rn);
+    return rho * e.yps[(size_t)i];
+  }
+  const double qa = 1.0, qb = -2 * half, qc = rmax2;  // R^2 = qa t^2 + qb t + qc
+  const double t2 = dist_cm / kKm2Cm;
+  const double t1 = 0.;
+  auto sq_int = [&](double t) {
+    return qa * std::pow(t, 3) / 3. + qb * std::pow(t, 2) / 2. + qc * t;
+  };
+  auto lin_a = [&](double t) {
+    return (qb + 2. * qa * t) * std::sqrt(qc + t * (qb + qa * t)) / (4. * qa);
+  };
+  auto lin_b = [&](double t) {
+    return (qb * qb - 4. * qa * qc) *
+           std::log(qb + 2. * qa * t + 2. * std::sqrt(qa) * std::sqrt(qc + t * (qb + qa * t))) /
+           (8. * std::pow(qa, 3. / 2.));
+  };
+  const double sq = sq_int(t2) - sq_int(t1);
+  const double lin = (lin_a(t2) - lin_b(t2)) - (lin_a(t1) - lin_b(t1));
+  const double con = t2 - t1;
+  const double rho = (A * sq + B * lin + Cc * con) / (t2 - t1);
+  return rho * e.yps[(size_t)i];
+}
 
 // atmoscpupropagator.hpp:116-170.
 void set_density(Earth& e, const std::vector<double>& r,
@@ -369,6 +425,45 @@ void set_density(Earth& e, const std::vector<double>& r,
 
 // atmoscpupropagator.hpp:431-444 without the MaxNLayers cap: number of shell
 // rows whose cosine limit lies strictly above c.
+// atmoscpupropagator.hpp:181-245: the polynomial model, flipped to
+// descending rows like the constant one (rhos untouched).
+void set_density_poly(Earth& e, const std::vector<double>& r,
+                      const std::vector<double>& a, const std::vector<double>& b,
+                      const std::vector<double>& c, const std::vector<double>& ye) {
+  if (a.size() != r.size())
+    throw std::runtime_error("setDensity : a.size() != radii.size()");
+  if (a.size() != ye.size())
+    throw std::runtime_error("setDensity : a.size() != yps.size()");
+  if (a.empty() || b.empty() || c.empty() || r.empty() || ye.empty())
+    throw std::runtime_error("setDensity : vectors must not be empty");
+  if (b.size() != r.size() || c.size() != r.size())
+    throw std::runtime_error("setDensity : b and c must have radii.size() entries");
+  bool flip = false;
+  if (r.size() >= 2) {
+    const int sign = (r[1] - r[0] > 0 ? 1 : -1);
+    for (size_t i = 1; i < r.size(); ++i)
+      if ((r[i] - r[i - 1]) * sign < 0)
+        throw std::runtime_error("radii order messed up");
+    flip = (sign == 1);
+  }
+  e.radii = r;
+  e.as = a;
+  e.bs = b;
+  e.cs = c;
+  e.yps = ye;
+  if (flip)
+    for (std::vector<double>* v : {&e.radii, &e.yps, &e.as, &e.bs, &e.cs})
+      std::reverse(v->begin(), v->end());
+  e.coslimit.clear();
+  for (size_t i = 0; i < e.radii.size(); ++i) {
+    double x = -1 * std::sqrt(1 - (e.radii[i] * e.radii[i] /
+                                   (kREarthKm * kREarthKm)));
+    if (i == 0) x = 0;
+    e.coslimit.push_back(x);
+  }
+  e.poly = true;
+}
+
 int max_layer(const Earth& e, double c) {
   int n = 0;
   for (double lim : e.coslimit)
@@ -390,8 +485,8 @@ static std::vector<std::string> split_fields(std::string line) {
   return f;
 }
 
-// atmoscpupropagator.hpp:250-344 for the 3-column (radius, rho, Ye) format.
-// The 5-column polynomial format is outside this oracle's scope and raises.
+// atmoscpupropagator.hpp:250-344: 3 columns (radius, rho, Ye) or 5 columns
+// (radius, a, b, c, Ye; fields converted with atof as :313-327 does).
 void set_density_file(Earth& e, const std::string& path) {
   std::ifstream in(path);
   if (!in) throw std::runtime_error("could not open density file " + path);
@@ -402,11 +497,26 @@ void set_density_file(Earth& e, const std::string& path) {
     const int n = (int)split_fields(line).size();
     if (ncol == 0) ncol = n;
   }
-  if (ncol != 3)
+  if (ncol != 3 && ncol != 5)
     throw std::runtime_error("unsupported Earth model: " +
                              std::to_string(ncol) + " entries per line");
   in.clear();
   in.seekg(0);
+  if (ncol == 5) {
+    std::vector<double> r, a, b, c, y;
+    while (std::getline(in, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      const std::vector<std::string> f = split_fields(line);
+      if (f.size() < 5) throw std::runtime_error("short line in " + path);
+      r.push_back(std::atof(f[0].c_str()));
+      a.push_back(std::atof(f[1].c_str()));
+      b.push_back(std::atof(f[2].c_str()));
+      c.push_back(std::atof(f[3].c_str()));
+      y.push_back(std::atof(f[4].c_str()));
+    }
+    set_density_poly(e, r, a, b, c, y);
+    return;
+  }
   std::vector<double> r, d, y;
   double a, b, c;
   while (in >> a >> b >> c) {
@@ -417,8 +527,9 @@ void set_density_file(Earth& e, const std::string& path) {
   set_density(e, r, d, y);
 }
 
-// physics.hpp:789-1150 (calculate_atmos, constant-density, no production
-// height averaging).  out is [cos][E][9], channel a*3+b = P(a->b).
+// physics.hpp:789-1150 (calculate_atmos, no production height averaging;
+// constant or polynomial densities, :915-920).  out is [cos][E][9], channel
+// a*3+b = P(a->b).
 void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
                      const double* cosv, int nC, const double* ev, int nE,
                      double h_cm, double* out) {
@@ -445,7 +556,8 @@ void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
         const double dist = segment_cm(earth.radii.data(), nrows, layer, M,
                                        path_cm, earth_cm, cz);
         const double rye =
-            layer_rho_ye(earth.rhos.data(), earth.yps.data(), layer, M);
+            earth.poly ? layer_rho_ye_poly(earth, cz, layer, M, dist)
+                       : layer_rho_ye(earth.rhos.data(), earth.yps.data(), layer, M);
         cx C[3][3][3];
         double arg[3];
         layer_expansion(o, E, rye, dist / kKm2Cm, anti, C, arg);
@@ -586,6 +698,65 @@ int orc_atmos_f64(int nutype, const double* cosv, int nC, const double* ev,
     orc::set_threads(threads);
     orc::calculate_atmos(o, e, nutype != 0, cosv, nC, ev, nE,
                          h_km * 100000.0, out);
+  } catch (const std::exception& ex) {
+    orc::put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// The same with a polynomial Earth model (setDensity(radii, a, b, c, yps)).
+int orc_atmos_poly_f64(int nutype, const double* cosv, int nC, const double* ev,
+                       int nE, const double* radii, const double* pa,
+                       const double* pb, const double* pc, const double* yps,
+                       int nrows, double th12, double th13, double th23,
+                       double dcp, int kNuType, double dm12, double dm23,
+                       double h_km, int threads, double* out, char* err,
+                       int errlen) {
+  try {
+    orc::Osc o;
+    orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+    orc::Earth e;
+    auto v = [&](const double* x) { return std::vector<double>(x, x + nrows); };
+    orc::set_density_poly(e, v(radii), v(pa), v(pb), v(pc), v(yps));
+    orc::set_threads(threads);
+    orc::calculate_atmos(o, e, nutype != 0, cosv, nC, ev, nE,
+                         h_km * 100000.0, out);
+  } catch (const std::exception& ex) {
+    orc::put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// Probe: rho*Ye of every (cosine, layer 1..M) of a polynomial model, as the
+// propagation uses it; out[c * nrows + l - 1], 0 past M.
+int orc_poly_rye(const double* cosv, int nC, const double* radii, const double* pa,
+                 const double* pb, const double* pc, const double* yps, int nrows,
+                 double h_km, double* out, char* err, int errlen) {
+  try {
+    orc::Earth e;
+    auto v = [&](const double* x) { return std::vector<double>(x, x + nrows); };
+    orc::set_density_poly(e, v(radii), v(pa), v(pb), v(pc), v(yps));
+    const double h_cm = h_km * 100000.0;
+    for (int ic = 0; ic < nC; ++ic) {
+      const double cz = cosv[ic];
+      const int M = orc::max_layer(e, cz);
+      const double earth_cm = -2.0 * cz * orc::kREarthCm;
+      const double path_cm =
+          std::sqrt((orc::kREarthCm + h_cm) * (orc::kREarthCm + h_cm) -
+                    (orc::kREarthCm * orc::kREarthCm) * (1 - cz * cz)) -
+          orc::kREarthCm * cz;
+      for (int l = 1; l <= nrows; ++l) {
+        double r = 0.0;
+        if (l <= M) {
+          const double dist = orc::segment_cm(e.radii.data(), nrows, l, M, path_cm,
+                                              earth_cm, cz);
+          r = orc::layer_rho_ye_poly(e, cz, l, M, dist);
+        }
+        out[(size_t)ic * nrows + (size_t)(l - 1)] = r;
+      }
+    }
   } catch (const std::exception& ex) {
     orc::put_err(err, errlen, ex.what());
     return -1;

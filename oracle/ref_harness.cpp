@@ -75,11 +75,13 @@ std::vector<T> to_vec(const double* p, int n) {
 // Runs the reference atmospheric CPU propagator in the call order of
 // example/mainatmos.cpp:121-148 (energies, cosines, MNS, masses, density,
 // production height, calculate).  `path` selects setDensityFromFile instead of
-// setDensity(radii, rhos, yps).
+// setDensity(radii, rhos, yps); `poly` (a, b, c per row) selects
+// setDensity(radii, a, b, c, yps).
 template <typename T, typename OUT>
 int run_atmos(int nutype, const double* cosv, int nC, const double* ev, int nE,
               const double* radii, const double* rhos, const double* yps,
-              int nrows, const char* path, double th12, double th13,
+              int nrows, const char* path, const double* const* poly, double th12,
+              double th13,
               double th23, double dcp, int kNuType, double dm12, double dm23,
               double h_km, int threads, OUT* out, char* err, int errlen) {
   using namespace cudaprob3linear;
@@ -91,6 +93,10 @@ int run_atmos(int nutype, const double* cosv, int nC, const double* ev, int nE,
     p.setNeutrinoMasses((T)dm12, (T)dm23);
     if (path) {
       p.setDensityFromFile(std::string(path));
+    } else if (poly) {
+      p.setDensity(to_vec<T>(radii, nrows), to_vec<T>(poly[0], nrows),
+                   to_vec<T>(poly[1], nrows), to_vec<T>(poly[2], nrows),
+                   to_vec<T>(yps, nrows));
     } else {
       p.setDensity(to_vec<T>(radii, nrows), to_vec<T>(rhos, nrows),
                    to_vec<T>(yps, nrows));
@@ -150,8 +156,8 @@ int ref_atmos_f64(int nutype, const double* cosv, int nC, const double* ev,
                   double dm23, double h_km, int threads, double* out,
                   char* err, int errlen) {
   return run_atmos<double>(nutype, cosv, nC, ev, nE, radii, rhos, yps, nrows,
-                           nullptr, th12, th13, th23, dcp, kNuType, dm12, dm23,
-                           h_km, threads, out, err, errlen);
+                           nullptr, nullptr, th12, th13, th23, dcp, kNuType, dm12,
+                           dm23, h_km, threads, out, err, errlen);
 }
 
 int ref_atmos_f32(int nutype, const double* cosv, int nC, const double* ev,
@@ -161,8 +167,8 @@ int ref_atmos_f32(int nutype, const double* cosv, int nC, const double* ev,
                   double dm23, double h_km, int threads, float* out, char* err,
                   int errlen) {
   return run_atmos<float>(nutype, cosv, nC, ev, nE, radii, rhos, yps, nrows,
-                          nullptr, th12, th13, th23, dcp, kNuType, dm12, dm23,
-                          h_km, threads, out, err, errlen);
+                          nullptr, nullptr, th12, th13, th23, dcp, kNuType, dm12,
+                          dm23, h_km, threads, out, err, errlen);
 }
 
 int ref_atmos_file_f64(int nutype, const double* cosv, int nC,
@@ -171,8 +177,21 @@ int ref_atmos_file_f64(int nutype, const double* cosv, int nC,
                        double dm12, double dm23, double h_km, int threads,
                        double* out, char* err, int errlen) {
   return run_atmos<double>(nutype, cosv, nC, ev, nE, nullptr, nullptr, nullptr,
-                           0, path, th12, th13, th23, dcp, kNuType, dm12, dm23,
-                           h_km, threads, out, err, errlen);
+                           0, path, nullptr, th12, th13, th23, dcp, kNuType, dm12,
+                           dm23, h_km, threads, out, err, errlen);
+}
+
+int ref_atmos_poly_f64(int nutype, const double* cosv, int nC, const double* ev,
+                       int nE, const double* radii, const double* pa,
+                       const double* pb, const double* pc, const double* yps,
+                       int nrows, double th12, double th13, double th23,
+                       double dcp, int kNuType, double dm12, double dm23,
+                       double h_km, int threads, double* out, char* err,
+                       int errlen) {
+  const double* const poly[3] = {pa, pb, pc};
+  return run_atmos<double>(nutype, cosv, nC, ev, nE, radii, nullptr, yps, nrows,
+                           nullptr, poly, th12, th13, th23, dcp, kNuType, dm12,
+                           dm23, h_km, threads, out, err, errlen);
 }
 
 int ref_beam_f64(int nutype, const double* ev, int nE, double rho, double L_km,

@@ -2,7 +2,7 @@
 propagators (oracle/_ref/libcp3ref*.so, compiled from the reference headers by
 oracle/Makefile).  Run in a container that has the reference sources:
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [--only-poly]
 
 Each fixture is an .npz of plain float64/int arrays (inputs and expected
 outputs, [cos][E][9] or [E][9], channel a*3+b = P(a->b)); manifest.json
@@ -35,6 +35,38 @@ def save(name, **arrays):
     path = os.path.join(HERE, name + ".npz")
     np.savez(path, **arrays)
     return os.path.getsize(path)
+
+
+def poly_golden(ref, manifest):
+    """G6: the quadratic-density 4-shell model through
+    setDensity(radii, a, b, c, yps), 32 x 32 points (cos from -1, the
+    straight-through branch included), nominal parameters, nu and nu-bar."""
+    ep = W.prem4_poly()
+    cz = W.cos_grid(32)
+    ev = W.log_grid(32, 0.1, 100.0)
+    for nt, tag in ((0, "nu"), (1, "nubar")):
+        out = ref.atmos_poly(nt, cz, ev, ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                             osc_of(W.NOMINAL, nt), W.PROD_HEIGHT_KM)
+        name = f"g6_prem4poly_{tag}"
+        size = save(name, cosv=cz, ev=ev, radii=ep.radii, a=ep.a, b=ep.b,
+                    c=ep.c, yps=ep.yps, prob=out, nutype=np.int64(nt))
+        manifest["fixtures"][name] = {
+            "workload": "4 quadratic-density shells, 32x32, cos -1+i/32, "
+                        "E 0.1-100 GeV log, h 22 km",
+            "nutype": nt, "lib": "libcp3ref.so", "bytes": size}
+
+
+def only_poly():
+    """Adds G6 to an existing set without rewriting the other fixtures."""
+    ref = load_ref(deep=False)
+    if ref is None:
+        sys.exit("oracle/_ref is not built (needs the reference sources)")
+    path = os.path.join(HERE, "manifest.json")
+    with open(path) as f:
+        manifest = json.load(f)
+    poly_golden(ref, manifest)
+    with open(path, "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
 
 
 def main():
@@ -176,6 +208,8 @@ def main():
             "workload": "models/PREM_4layer.dat via setDensityFromFile, 20x20",
             "lib": "libcp3ref.so", "bytes": size}
 
+    poly_golden(ref, manifest)
+
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
     total = sum(v["bytes"] for v in manifest["fixtures"].values())
@@ -183,4 +217,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--only-poly"]:
+        only_poly()
+    else:
+        main()

@@ -259,3 +259,91 @@ def test_oracle_errors(oracle):
         oracle.atmos_file(0, [-0.5], [1.0], "/nonexistent.dat",
                           osc_for(NOM, 0), 22.0)
     del e
+
+
+# ------------------------------------------------------ polynomial densities
+
+def poly_model():
+    return W.prem4_poly()
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_poly_golden(oracle, nutype):
+    """setDensity(radii, a, b, c, yps) against the compiled reference's own
+    polynomial path (tests/golden/g6_*), cos = -1 (straight through the
+    centre) included."""
+    g = golden(f"g6_prem4poly_{'nubar' if nutype else 'nu'}")
+    out = oracle.atmos_poly(nutype, g["cosv"], g["ev"], g["radii"], g["a"],
+                            g["b"], g["c"], g["yps"], osc_for(NOM, nutype),
+                            22.0, threads=4)
+    assert np.abs(out - g["prob"]).max() <= TOL_GOLD
+
+
+def test_poly_file_and_reference(oracle, tmp_path):
+    """The 5-column density file (radius, a, b, c, Ye) loads the same model;
+    where the compiled reference is built, its file and array paths agree
+    with the oracle's bitwise, also for drawn parameter sets."""
+    ep = poly_model()
+    path = tmp_path / "poly.dat"
+    ep.write(str(path))
+    cz, ev = W.cos_grid(24), W.log_grid(24, 0.1, 100.0)
+    a = oracle.atmos_poly(0, cz, ev, ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                          osc_for(NOM, 0), 22.0)
+    assert np.array_equal(oracle.atmos_file(0, cz, ev, str(path), osc_for(NOM, 0), 22.0), a)
+    from oracle.loader import load_ref
+    ref = load_ref()
+    if ref is None:
+        return
+    assert np.array_equal(ref.atmos_file(0, cz, ev, str(path), osc_for(NOM, 0), 22.0), a)
+    for p in W.draw_param_sets(2, 606):
+        row = [p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+        for nt in (0, 1):
+            x = oracle.atmos_poly(nt, cz, ev, ep.radii, ep.a, ep.b, ep.c,
+                                  ep.yps, osc_for(row, nt), 22.0)
+            y = ref.atmos_poly(nt, cz, ev, ep.radii, ep.a, ep.b, ep.c,
+                               ep.yps, osc_for(row, nt), 22.0)
+            assert np.array_equal(x, y)
+
+
+def test_poly_average_is_the_segment_mean(oracle):
+    """Independently of the reference's closed forms: each (cosine, layer)
+    density equals the mean of rho(R(t)) * Ye over the layer's segment by
+    64-point Gauss-Legendre quadrature along the chord (R(t)^2 = t^2 - 2 d t
+    + R_i^2 from the shell's outer boundary, d its half chord), and the
+    radial mean over the shell at cos = -1."""
+    ep = poly_model()
+    desc = lambda v: np.asarray(v)[::-1]  # noqa: E731  rows as the model holds them
+    radii, A, B, Cc, ye = map(desc, (ep.radii, ep.a, ep.b, ep.c, ep.yps))
+    cz = np.array([-1.0, -0.999, -0.97, -0.9, -0.75, -0.6, -0.45, -0.3, -0.1, -0.02])
+    got = oracle.poly_rye(cz, ep.radii, ep.a, ep.b, ep.c, ep.yps, 22.0)
+    x, w = np.polynomial.legendre.leggauss(64)
+    R = W.R_EARTH_KM
+    for ic, c in enumerate(cz):
+        M, segs, _ = oracle.segments(c, ep.radii, ep.c, ep.yps, 22.0)
+        rmin = R * np.sqrt(1 - c * c)
+        for l in range(1, M + 1):
+            i = l - 1
+            if c == -1.0:
+                lo, hi = radii[i + 1] if i + 1 < len(radii) else 0.0, radii[i]
+                r = 0.5 * (hi - lo) * x + 0.5 * (hi + lo)
+                mean = 0.5 * np.sum(w * (A[i] * r * r + B[i] * r + Cc[i]))
+            else:
+                t2 = segs[l]
+                d = np.sqrt(radii[i] ** 2 - rmin ** 2)
+                t = 0.5 * t2 * (x + 1)
+                r = np.sqrt(np.maximum(t * t - 2 * d * t + radii[i] ** 2, 0.0))
+                mean = 0.5 * np.sum(w * (A[i] * r * r + B[i] * r + Cc[i]))
+            assert abs(got[ic, i] - mean * ye[i]) <= 1e-9 * abs(mean * ye[i]), (c, l)
+        assert np.all(got[ic, M:] == 0.0)
+
+
+def test_poly_constant_profile_equals_constant_model(oracle):
+    """a = b = 0 reduces the polynomial path to the constant-density one."""
+    e = W.prem4()
+    z = np.zeros_like(e.rhos)
+    cz, ev = W.cos_grid(24), W.log_grid(24, 0.1, 100.0)
+    for nt in (0, 1):
+        x = oracle.atmos_poly(nt, cz, ev, e.radii, z, z, e.rhos, e.yps,
+                              osc_for(NOM, nt), 22.0)
+        y = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(NOM, nt), 22.0)
+        assert np.abs(x - y).max() < 1e-13
