@@ -103,7 +103,7 @@ struct cp3_atmos {
   int n_layers = 0;
   DevBuf d_ev, d_perm, d_M, d_Lvac, d_seg, d_rye, d_out;
   PinnedBuf h_out;
-  int S = 0;
+  int S = 0, rye_cs = 0;
   bool table_dirty = true, energies_dirty = true, results_on_host = false;
 
   ~cp3_atmos() {
@@ -126,6 +126,7 @@ struct cp3_atmos {
       cp3::TrajTable tt;
       cp3::build_traj_table(tt, earth, cosines.data(), nC, h_cm);
       S = tt.S;
+      rye_cs = tt.rye_cs;
       d_perm.upload(tt.perm.data(), tt.perm.size() * sizeof(int32_t), dev);
       d_M.upload(tt.M.data(), tt.M.size() * sizeof(int32_t), dev);
       d_Lvac.upload(tt.Lvac.data(), tt.Lvac.size() * sizeof(double), dev);
@@ -146,6 +147,7 @@ struct cp3_atmos {
     g.nC = nC;
     g.nE = nE;
     g.S = S;
+    g.rye_cs = rye_cs;
     return g;
   }
 
@@ -162,11 +164,15 @@ struct cp3_atmos {
     job.unit_begin = 0;
     job.etiles = cp3::energy_tiles(job.g);
     results_on_host = false;
-    if (solve_mode == CP3_SOLVE_SHARED) {
+    // a polynomial model's densities differ per cosine: the shared solve with
+    // one cosine per wave and one wave per workgroup solves per trajectory
+    const bool per_cosine = job.g.rye_cs != 0;
+    if (solve_mode == CP3_SOLVE_SHARED || per_cosine) {
       int nc = 1, w = 1;
       auto_tuning(precision, job.g, &nc, &w);
       if (nc_force) nc = nc_force;
       if (w_force) w = w_force;
+      if (per_cosine) nc = w = 1;
       hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
       hip_check(cp3::launch_atmos_single(precision, nc, w, job, stream),
                 "atmospheric kernel launch");
@@ -404,13 +410,36 @@ int cp3_atmos_set_density(cp3_atmos* p, const double* r, const double* rho,
   });
 }
 
+int cp3_atmos_set_density_poly(cp3_atmos* p, const double* r, const double* a,
+                               const double* b, const double* c, const double* ye,
+                               int n) {
+  return guard([&] {
+    need(p);
+    if (n < 0) throw Error(CP3_ERR_INVALID, "setDensity : negative size");
+    std::vector<double> vr, va, vb, vc, vy;
+    if (n > 0) {
+      need_ptr(r, "radii");
+      need_ptr(a, "a");
+      need_ptr(b, "b");
+      need_ptr(c, "c");
+      need_ptr(ye, "yps");
+      vr.assign(r, r + n);
+      va.assign(a, a + n);
+      vb.assign(b, b + n);
+      vc.assign(c, c + n);
+      vy.assign(ye, ye + n);
+    }
+    cp3::earth_set_poly(p->earth, vr, va, vb, vc, vy);
+    p->n_layers = (int)p->earth.radii.size();
+    p->table_dirty = true;
+  });
+}
+
 int cp3_atmos_set_density_from_file(cp3_atmos* p, const char* path) {
   return guard([&] {
     need(p);
     need_ptr(path, "path");
-    std::vector<double> r, d, y;
-    cp3::earth_read_file(path, r, d, y);
-    cp3::earth_set(p->earth, r, d, y);
+    cp3::earth_load_file(p->earth, path);
     p->n_layers = (int)p->earth.radii.size();
     p->table_dirty = true;
   });

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <fstream>
 #include <numeric>
 #include <stdexcept>
@@ -150,11 +151,24 @@ inline SetConst make_set(double t12, double t13, double t23, double dcp,
 }
 
 // Earth model after setDensity (atmoscpupropagator.hpp:116-170): rows
-// descending by radius, coslimit per row.
+// descending by radius, coslimit per row.  A polynomial model
+// (setDensity(radii, a, b, c, yps), :181-245) gives row i the density
+// as[i] R^2 + bs[i] R + cs[i] at radius R km instead of rhos[i].
 struct EarthModel {
   std::vector<double> radii, rhos, yps, coslimit;
-  bool set = false;
+  std::vector<double> as, bs, cs;
+  bool set = false, poly = false;
 };
+
+inline void earth_coslimits(EarthModel& e) {
+  e.coslimit.clear();
+  for (size_t i = 0; i < e.radii.size(); ++i) {
+    double x = -1 * std::sqrt(1 - (e.radii[i] * e.radii[i] /
+                                   (kREarthKm * kREarthKm)));
+    if (i == 0) x = 0;
+    e.coslimit.push_back(x);
+  }
+}
 
 inline void earth_set(EarthModel& e, const std::vector<double>& r,
                       const std::vector<double>& rho,
@@ -181,14 +195,46 @@ inline void earth_set(EarthModel& e, const std::vector<double>& r,
     std::reverse(e.rhos.begin(), e.rhos.end());
     std::reverse(e.yps.begin(), e.yps.end());
   }
-  e.coslimit.clear();
-  for (size_t i = 0; i < e.radii.size(); ++i) {
-    double x = -1 * std::sqrt(1 - (e.radii[i] * e.radii[i] /
-                                   (kREarthKm * kREarthKm)));
-    if (i == 0) x = 0;
-    e.coslimit.push_back(x);
-  }
+  earth_coslimits(e);
   e.set = true;
+  e.poly = false;
+}
+
+// setDensity(radii, a, b, c, yps) (atmoscpupropagator.hpp:181-245): the same
+// checks and flip, rhos untouched.  b and c must be as long as the radii (the
+// reference checks only a and would read past shorter lists).
+inline void earth_set_poly(EarthModel& e, const std::vector<double>& r,
+                           const std::vector<double>& a,
+                           const std::vector<double>& b,
+                           const std::vector<double>& c,
+                           const std::vector<double>& ye) {
+  if (a.size() != r.size())
+    throw std::invalid_argument("setDensity : a.size() != radii.size()");
+  if (a.size() != ye.size())
+    throw std::invalid_argument("setDensity : a.size() != yps.size()");
+  if (a.empty() || b.empty() || c.empty() || r.empty() || ye.empty())
+    throw std::invalid_argument("setDensity : vectors must not be empty");
+  if (b.size() != r.size() || c.size() != r.size())
+    throw std::invalid_argument("setDensity : b.size() and c.size() must equal radii.size()");
+  bool flip = false;
+  if (r.size() >= 2) {
+    const int sign = (r[1] - r[0] > 0 ? 1 : -1);
+    for (size_t i = 1; i < r.size(); ++i)
+      if ((r[i] - r[i - 1]) * sign < 0)
+        throw std::invalid_argument("radii order messed up");
+    flip = (sign == 1);
+  }
+  e.radii = r;
+  e.as = a;
+  e.bs = b;
+  e.cs = c;
+  e.yps = ye;
+  if (flip)
+    for (std::vector<double>* v : {&e.radii, &e.yps, &e.as, &e.bs, &e.cs})
+      std::reverse(v->begin(), v->end());
+  earth_coslimits(e);
+  e.set = true;
+  e.poly = true;
 }
 
 // setChemicalComposition (atmoscpupropagator.hpp:481-490): the new list is
@@ -223,12 +269,25 @@ inline int density_file_fields(std::string line) {
   return n + 1;
 }
 
-// setDensityFromFile for the 3-column (radius km, density g/cm^3, Ye) format
-// (atmoscpupropagator.hpp:250-301).  Other formats raise std::invalid_argument
-// (the reference ends the process there, :338-342).
-inline void earth_read_file(const std::string& path, std::vector<double>& r,
-                            std::vector<double>& rho,
-                            std::vector<double>& ye) {
+// The fields of a density-file line, split as density_file_fields counts them
+// (atmoscpupropagator.hpp:313-322).
+inline std::vector<std::string> density_file_split(std::string line) {
+  std::vector<std::string> f;
+  while (line.find_first_of(" ") != std::string::npos) {
+    size_t pos = line.find_first_of(" ");
+    f.push_back(line.substr(0, pos));
+    while (pos + 1 < line.size() && line[pos] == line[pos + 1]) ++pos;
+    line = line.substr(pos + 1);
+  }
+  f.push_back(line);
+  return f;
+}
+
+// setDensityFromFile (atmoscpupropagator.hpp:250-344): 3 columns (radius km,
+// density g/cm^3, Ye) or 5 columns (radius, a, b, c, Ye: a polynomial model,
+// fields converted with atof as :313-327).  Other formats raise
+// std::invalid_argument (the reference ends the process there, :338-342).
+inline void earth_load_file(EarthModel& e, const std::string& path) {
   std::ifstream in(path);
   if (!in) throw std::invalid_argument("could not open density file " + path);
   std::string line;
@@ -238,21 +297,35 @@ inline void earth_read_file(const std::string& path, std::vector<double>& r,
     const int n = density_file_fields(line);
     if (ncol == 0) ncol = n;
   }
-  if (ncol != 3)
+  if (ncol != 3 && ncol != 5)
     throw std::invalid_argument("unsupported Earth model in " + path + ": " +
                                 std::to_string(ncol) +
-                                " entries per line (3 expected)");
+                                " entries per line (3 or 5 expected)");
   in.clear();
   in.seekg(0);
-  r.clear();
-  rho.clear();
-  ye.clear();
+  if (ncol == 5) {
+    std::vector<double> r, a, b, c, ye;
+    while (std::getline(in, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      const std::vector<std::string> f = density_file_split(line);
+      if (f.size() < 5) throw std::invalid_argument("short line in density file " + path);
+      r.push_back(std::atof(f[0].c_str()));
+      a.push_back(std::atof(f[1].c_str()));
+      b.push_back(std::atof(f[2].c_str()));
+      c.push_back(std::atof(f[3].c_str()));
+      ye.push_back(std::atof(f[4].c_str()));
+    }
+    earth_set_poly(e, r, a, b, c, ye);
+    return;
+  }
+  std::vector<double> r, rho, ye;
   double a, b, c;
   while (in >> a >> b >> c) {
     r.push_back(a);
     rho.push_back(b);
     ye.push_back(c);
   }
+  earth_set(e, r, rho, ye);
 }
 
 // Trajectory table of one (Earth model, cosine list, production height):
@@ -260,11 +333,14 @@ inline void earth_read_file(const std::string& path, std::vector<double>& r,
 //   Lvac[c]     atmosphere segment, km (physics.hpp:757-759);
 //   seg[c*S+i]  km in shell i for i < M[c]: half chord difference for
 //               i < M[c]-1, full chord for the innermost (physics.hpp:780-786);
-//   rye[i]      rho*Ye of shell i (physics.hpp:634-646);
+//   rye[i]      rho*Ye of shell i (physics.hpp:634-646); for a polynomial
+//               model rye[c*S+i], the density averaged over that segment
+//               (physics.hpp:653-739), and rye_cs = S (0 otherwise): row
+//               stride of rye per cosine;
 //   perm        cosine indices ordered by decreasing M (stable), so blocks of
 //               neighbouring entries cross similar numbers of shells.
 struct TrajTable {
-  int nC = 0, S = 0;
+  int nC = 0, S = 0, rye_cs = 0;
   std::vector<int32_t> M, perm;
   std::vector<double> Lvac, seg, rye;
 };
@@ -288,6 +364,53 @@ inline double segment_cm(const std::vector<double>& radii, int layer,
   return cross_this * kKm2Cm;
 }
 
+// rho*Ye of layer `layer` of a trajectory at cosine cz through a polynomial
+// model, averaged over its segment of dist_cm (physics.hpp:653-739): the
+// segment runs from the shell's outer boundary inward, R(t)^2 = t^2 - 2 d t +
+// R_i^2 with d the half chord at R_i, and the integrals of R^2, R and 1 over
+// t in [0, dist] take the reference's closed forms in its operation order
+// (the results must agree to the last bit).  Straight through the centre
+// (cz = -1) it is the radial mean over the shell.
+inline double poly_layer_rye(const EarthModel& e, double cz, int layer, int max_layer,
+                             double dist_cm) {
+  if (cz >= 0 || layer == 0) return 0.0;
+  const int i = (layer <= max_layer) ? layer - 1 : 2 * max_layer - layer - 1;
+  const size_t k = (size_t)i;
+  const double Ri = e.radii[k];
+  // a model without its 0-km row reads radius 0 below its last row
+  const double Rn = (k + 1 < e.radii.size()) ? e.radii[k + 1] : 0.0;
+  const double pa = e.as[k], pb = e.bs[k], pc = e.cs[k];
+  const double r2min = std::pow(kREarthKm, 2) * (1 - std::pow(cz, 2));
+  const double r2max = std::pow(Ri, 2);
+  const double d = std::sqrt(r2max - r2min);
+  if (r2min == 0) {
+    const double up = pa * std::pow(Ri, 3) / 3. + pb * std::pow(Ri, 2) / 2. + pc * Ri;
+    const double dn = pa * std::pow(Rn, 3) / 3. + pb * std::pow(Rn, 2) / 2. + pc * Rn;
+    double rho = up - dn;
+    rho /= (Ri - Rn);
+    return rho * e.yps[k];
+  }
+  // R^2 = u t^2 + v t + w
+  const double u = 1.0, v = -2 * d, w = r2max;
+  const double t1 = 0., t2 = dist_cm / kKm2Cm;
+  const auto r2_int = [&](double t) {
+    return u * std::pow(t, 3) / 3. + v * std::pow(t, 2) / 2. + w * t;
+  };
+  const auto r_int_a = [&](double t) {
+    return (v + 2. * u * t) * std::sqrt(w + t * (v + u * t)) / (4. * u);
+  };
+  const auto r_int_b = [&](double t) {
+    return (v * v - 4. * u * w) *
+           std::log(v + 2. * u * t + 2. * std::sqrt(u) * std::sqrt(w + t * (v + u * t))) /
+           (8. * std::pow(u, 3. / 2.));
+  };
+  const double i2 = r2_int(t2) - r2_int(t1);
+  const double i1 = (r_int_a(t2) - r_int_b(t2)) - (r_int_a(t1) - r_int_b(t1));
+  const double i0 = t2 - t1;
+  const double rho = (pa * i2 + pb * i1 + pc * i0) / (t2 - t1);
+  return rho * e.yps[k];
+}
+
 inline void build_traj_table(TrajTable& t, const EarthModel& e,
                              const double* cosv, int nC, double h_cm) {
   const int S = (int)e.radii.size();
@@ -296,8 +419,13 @@ inline void build_traj_table(TrajTable& t, const EarthModel& e,
   t.M.assign((size_t)nC, 0);
   t.Lvac.assign((size_t)nC, 0.0);
   t.seg.assign((size_t)nC * (size_t)S, 0.0);
-  t.rye.assign((size_t)S, 0.0);
-  for (int i = 0; i < S; ++i) t.rye[(size_t)i] = e.rhos[(size_t)i] * e.yps[(size_t)i];
+  t.rye_cs = e.poly ? S : 0;
+  if (e.poly) {
+    t.rye.assign((size_t)nC * (size_t)S, 0.0);
+  } else {
+    t.rye.assign((size_t)S, 0.0);
+    for (int i = 0; i < S; ++i) t.rye[(size_t)i] = e.rhos[(size_t)i] * e.yps[(size_t)i];
+  }
   for (int c = 0; c < nC; ++c) {
     const double cz = cosv[c];
     const int M = earth_max_layer(e, cz);
@@ -307,9 +435,12 @@ inline void build_traj_table(TrajTable& t, const EarthModel& e,
                                      (kREarthCm * kREarthCm) * (1 - cz * cz)) -
                            kREarthCm * cz;
     t.Lvac[(size_t)c] = segment_cm(e.radii, 0, M, path_cm, earth_cm, cz) / kKm2Cm;
-    for (int l = 1; l <= M; ++l)
-      t.seg[(size_t)c * S + (size_t)(l - 1)] =
-          segment_cm(e.radii, l, M, path_cm, earth_cm, cz) / kKm2Cm;
+    for (int l = 1; l <= M; ++l) {
+      const double dist_cm = segment_cm(e.radii, l, M, path_cm, earth_cm, cz);
+      t.seg[(size_t)c * S + (size_t)(l - 1)] = dist_cm / kKm2Cm;
+      if (e.poly)
+        t.rye[(size_t)c * S + (size_t)(l - 1)] = poly_layer_rye(e, cz, l, M, dist_cm);
+    }
   }
   t.perm.resize((size_t)nC);
   std::iota(t.perm.begin(), t.perm.end(), 0);

@@ -201,15 +201,17 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
   Slab<P> vac;
   if constexpr (W == 1) {
     (void)st;
+    // per-cosine densities (rye_cs = S) come with NC == 1
+    const double* rye = g.rye + (int64_t)g.rye_cs * cidx[0];
     int i = 0;
     for (; i < Mmin - 1; ++i) {
       Slab<P> s;
-      solve_slab<P>(sc, E, inv, (D)uload(g.rye, i), s);
+      solve_slab<P>(sc, E, inv, (D)uload(rye, i), s);
       middle_all<P, NC>(t, s, g, cidx, i);
     }
     for (; i < Mmax; ++i) {
       Slab<P> s;
-      solve_slab<P>(sc, E, inv, (D)uload(g.rye, i), s);
+      solve_slab<P>(sc, E, inv, (D)uload(rye, i), s);
       apply_shell<P, NC>(t, s, g, cidx, Mc, i);
     }
     solve_slab<P>(sc, E, inv, D(0), vac);

@@ -21,8 +21,11 @@ struct GridDev {
   const int32_t* M;     // [nC] shells crossed
   const double* Lvac;   // [nC] atmosphere segment, km
   const double* seg;    // [nC * S] km per shell
-  const double* rye;    // [S] rho * Ye per shell
-  int32_t nC, nE, S, pad;
+  const double* rye;    // [S] rho * Ye per shell, or [nC * S] (rye_cs = S)
+  // row stride of rye per cosine: 0, or S for a polynomial model's
+  // per-trajectory densities (shared solve mode with cosine block 1 and one
+  // wave per workgroup only: those solve per cosine)
+  int32_t nC, nE, S, rye_cs;
 };
 
 // One atmospheric evaluation: a parameter set and neutrino type on a grid.

@@ -31,7 +31,7 @@ void atmos_point(const cp3::SetConst& sc, const cp3::TrajTable& tt, int c,
   cp3::traj_init<P>(t, M == 0);
   for (int i = 0; i < M; ++i) {
     cp3::Slab<P> s;
-    cp3::solve_slab<P>(sc, e, inv, (D)tt.rye[(size_t)i], s);
+    cp3::solve_slab<P>(sc, e, inv, (D)tt.rye[(size_t)tt.rye_cs * c + (size_t)i], s);
     const D L = (D)tt.seg[(size_t)c * tt.S + (size_t)i];
     if (i < M - 1)
       cp3::traj_middle<P>(t, s, L);
@@ -278,6 +278,53 @@ int cp3m_atmos(int precision, int nutype, const double* cosv, int nC,
       run_atmos<cp3::PolicyMixed>(sc, tt, ev, nE, out);
   } catch (const std::exception& ex) {
     put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// The same through a polynomial model (setDensity(radii, a, b, c, yps)).
+int cp3m_atmos_poly(int precision, int nutype, const double* cosv, int nC,
+                    const double* ev, int nE, const double* radii, const double* pa,
+                    const double* pb, const double* pc, const double* yps, int nrows,
+                    double th12, double th13, double th23, double dcp, int kNuType,
+                    double dm12, double dm23, double h_km, double* out, char* err,
+                    int errlen) {
+  try {
+    auto v = [&](const double* x) { return std::vector<double>(x, x + nrows); };
+    cp3::EarthModel em;
+    cp3::earth_set_poly(em, v(radii), v(pa), v(pb), v(pc), v(yps));
+    cp3::TrajTable tt;
+    cp3::build_traj_table(tt, em, cosv, nC, h_km * 100000.0);
+    const cp3::SetConst sc =
+        cp3::make_set(th12, th13, th23, dcp, kNuType, dm12, dm23, nutype);
+    if (precision == 0)
+      run_atmos<cp3::PolicyF64>(sc, tt, ev, nE, out);
+    else if (precision == 1)
+      run_atmos<cp3::PolicyF32>(sc, tt, ev, nE, out);
+    else
+      run_atmos<cp3::PolicyMixed>(sc, tt, ev, nE, out);
+  } catch (const std::exception& ex) {
+    put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// The per-(cosine, layer) densities of a polynomial model's trajectory table,
+// out[c * nrows + i] (0 past the cosine's innermost layer).
+int cp3m_poly_rye(const double* cosv, int nC, const double* radii, const double* pa,
+                  const double* pb, const double* pc, const double* yps, int nrows,
+                  double h_km, double* out) {
+  try {
+    auto v = [&](const double* x) { return std::vector<double>(x, x + nrows); };
+    cp3::EarthModel em;
+    cp3::earth_set_poly(em, v(radii), v(pa), v(pb), v(pc), v(yps));
+    cp3::TrajTable tt;
+    cp3::build_traj_table(tt, em, cosv, nC, h_km * 100000.0);
+    if (tt.rye_cs != tt.S) return -1;
+    for (size_t k = 0; k < tt.rye.size(); ++k) out[k] = tt.rye[k];
+  } catch (...) {
     return -1;
   }
   return 0;

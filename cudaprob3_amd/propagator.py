@@ -79,7 +79,27 @@ class AtmosPropagator:
         N.check(self._lib.cp3_atmos_set_neutrino_masses(
             self._h, float(dm12sq), float(dm23sq)))
 
-    def setDensity(self, radii, rhos, yps):
+    def setDensity(self, radii, *cols):
+        """setDensity(radii, rhos, yps) or, for a polynomial model,
+        setDensity(radii, a, b, c, yps) (density a R^2 + b R + c per shell,
+        R in km, averaged per trajectory segment)."""
+        if len(cols) == 4:
+            r, a, b, c, y = [_f64(x) for x in (radii,) + cols]
+            if len(a) != len(r) or len(a) != len(y):
+                msg = ("setDensity : a.size() != radii.size()"
+                       if len(a) != len(r) else
+                       "setDensity : a.size() != yps.size()")
+                raise Cp3Error(N.CP3_ERR_INVALID, msg)
+            if len(b) != len(r) or len(c) != len(r):
+                raise Cp3Error(N.CP3_ERR_INVALID, "setDensity : b.size() and "
+                               "c.size() must equal radii.size()")
+            N.check(self._lib.cp3_atmos_set_density_poly(
+                self._h, _dptr(r), _dptr(a), _dptr(b), _dptr(c), _dptr(y), len(r)))
+            return
+        if len(cols) != 2:
+            raise TypeError("setDensity(radii, rhos, yps) or "
+                            "setDensity(radii, a, b, c, yps)")
+        rhos, yps = cols
         r, d, y = _f64(radii), _f64(rhos), _f64(yps)
         if not (len(r) == len(d) == len(y)):
             # the C ABI takes one length; mirror the reference's size checks

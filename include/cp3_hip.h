@@ -89,9 +89,19 @@ int cp3_atmos_set_neutrino_masses(cp3_atmos* p, double dm12sq, double dm23sq);
  * row belongs to the shell whose outer radius is that row.  No shell cap. */
 int cp3_atmos_set_density(cp3_atmos* p, const double* radii_km,
                           const double* rhos, const double* yps, int n);
-/* AtmosCpuPropagator::setDensityFromFile (atmoscpupropagator.hpp:250-344),
- * 3-column (radius, density, Ye) files; other formats return
- * CP3_ERR_INVALID. */
+/* AtmosCpuPropagator::setDensity(radii, a, b, c, yps)
+ * (atmoscpupropagator.hpp:181-245): the shell whose outer radius is row i has
+ * density a[i] R^2 + b[i] R + c[i] (g/cm^3, R in km), averaged along each
+ * trajectory segment (physics.hpp:653-739).  b and c need n entries too (the
+ * reference checks only a).  Such a model solves per trajectory and layer,
+ * in the shared solve mode with one cosine per lane whatever the tuning and
+ * solve mode asked for. */
+int cp3_atmos_set_density_poly(cp3_atmos* p, const double* radii_km, const double* a,
+                               const double* b, const double* c, const double* yps,
+                               int n);
+/* AtmosCpuPropagator::setDensityFromFile (atmoscpupropagator.hpp:250-344):
+ * 3-column (radius, density, Ye) or 5-column (radius, a, b, c, Ye: a
+ * polynomial model as above) files; other formats return CP3_ERR_INVALID. */
 int cp3_atmos_set_density_from_file(cp3_atmos* p, const char* path);
 /* setChemicalComposition (atmoscpupropagator.hpp:481-490): n must equal the
  * number of model rows; values in the stored (outermost first) order. */

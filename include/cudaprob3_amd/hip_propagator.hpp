@@ -123,13 +123,22 @@ class HipPropagatorSingle {
                               y = detail::to_double(yps);
     check(cp3_atmos_set_density(live(), r.data(), d.data(), y.data(), (int)r.size()));
   }
-  // atmoscpupropagator.hpp:181-242: per-trajectory polynomial densities are
-  // not provided by this build.
-  void setDensity(const std::vector<FLOAT_T>&, const std::vector<FLOAT_T>&,
-                  const std::vector<FLOAT_T>&, const std::vector<FLOAT_T>&,
-                  const std::vector<FLOAT_T>&) {
-    throw std::runtime_error(
-        "setDensity(radii, a, b, c, yps): polynomial density models are not supported");
+  // atmoscpupropagator.hpp:181-245: densities a R^2 + b R + c per shell,
+  // averaged per trajectory segment.
+  void setDensity(const std::vector<FLOAT_T>& radii, const std::vector<FLOAT_T>& a,
+                  const std::vector<FLOAT_T>& b, const std::vector<FLOAT_T>& c,
+                  const std::vector<FLOAT_T>& yps) {
+    if (a.size() != radii.size())
+      throw std::runtime_error("setDensity : a.size() != radii.size()");
+    if (a.size() != yps.size())
+      throw std::runtime_error("setDensity : a.size() != yps.size()");
+    const std::vector<double> r = detail::to_double(radii), va = detail::to_double(a),
+                              vb = detail::to_double(b), vc = detail::to_double(c),
+                              y = detail::to_double(yps);
+    if (vb.size() != r.size() || vc.size() != r.size())
+      throw std::runtime_error("setDensity : b.size() and c.size() must equal radii.size()");
+    check(cp3_atmos_set_density_poly(live(), r.data(), va.data(), vb.data(), vc.data(),
+                                     y.data(), (int)r.size()));
   }
   // atmoscpupropagator.hpp:347-357: a placeholder in the reference too.
   void setDensity(FLOAT_T rho) {
@@ -137,7 +146,7 @@ class HipPropagatorSingle {
     std::cout << "DUMMY FUNCTION: ATMOS class uses setDensity(radii, rhos, yps) "
                  "or setDensityFromFile(filename)" << std::endl;
   }
-  // atmoscpupropagator.hpp:250-344 (3-column files)
+  // atmoscpupropagator.hpp:250-344 (3- and 5-column files)
   void setDensityFromFile(const std::string& filename) {
     check(cp3_atmos_set_density_from_file(live(), filename.c_str()));
   }

@@ -4,11 +4,14 @@
 // for the parity test (tests/test_gpu_dropin.py).
 //
 //   dropin_main atmos <out.bin> <precision 0|1|2> <nutype 0|1>
+//   dropin_main poly  <out.bin> <precision 0|1|2> <nutype 0|1>
 //   dropin_main beam  <out.bin> <precision 0|1|2> <nutype 0|1>
 //   dropin_main errors
 //
 // atmos: the 4-shell model, 100 x 100 grid (cos -1 + i/100, E 0.1-100 GeV
 // log), nominal parameters, h = 22 km; output [cos][E][9] doubles.
+// poly: the same grid through setDensity(radii, a, b, c, yps) with the
+// quadratic profiles of cudaprob3_amd.workloads.prem4_poly.
 // beam: 1300 km, 2.6 g/cm^3, 500 energies over 0.1-20 GeV; output [E][9].
 #include <cmath>
 #include <cstdio>
@@ -33,7 +36,7 @@ static const double kTh13 = std::asin(std::sqrt(0.0220));
 static const double kTh23 = std::asin(std::sqrt(0.546));
 static const double kDcp = -1.6, kDm12 = 7.53e-5, kDm23 = 2.494e-3;
 
-static int run_atmos(const char* path, int precision, int nutype) {
+static int run_atmos(const char* path, int precision, int nutype, bool poly = false) {
   const int nC = 100, nE = 100;
   std::vector<double> cz((size_t)nC);
   for (int i = 0; i < nC; ++i) cz[(size_t)i] = -1.0 + i / 100.0;
@@ -43,8 +46,17 @@ static int run_atmos(const char* path, int precision, int nutype) {
   p.setCosineList(cz);
   p.setMNSMatrix(kTh12, kTh13, kTh23, kDcp, nutype ? -1 : 1);
   p.setNeutrinoMasses(kDm12, kDm23);
-  p.setDensity({0.0, 1220.0, 3480.0, 5701.0, 6371.0}, {13.0, 13.0, 11.3, 5.0, 3.3},
-               {0.5, 0.5, 0.5, 0.5, 0.5});
+  if (poly) {
+    const double R2 = 6371.0 * 6371.0, R = 6371.0;
+    p.setDensity({0.0, 1220.0, 3480.0, 5701.0, 6371.0},
+                 {-8.8381 / R2, -8.8381 / R2, -3.6426 / R2, 5.5283 / R2, 0.0 / R2},
+                 {0.0 / R, 0.0 / R, -1.2638 / R, -6.4761 / R, -1.4836 / R},
+                 {13.0885, 13.0885, 12.5815, 7.9565, 5.3197},
+                 {0.466, 0.466, 0.466, 0.494, 0.494});
+  } else {
+    p.setDensity({0.0, 1220.0, 3480.0, 5701.0, 6371.0}, {13.0, 13.0, 11.3, 5.0, 3.3},
+                 {0.5, 0.5, 0.5, 0.5, 0.5});
+  }
   p.setProductionHeight(22.0);
   p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
   std::vector<double> out((size_t)nC * nE * 9);
@@ -114,6 +126,8 @@ static int run_errors() {
                  "radii order");
   bad += !throws([&] { p.setDensity({0.0, 1.0}, {1.0}, {0.5, 0.5}); }, "rhos.size() != radii.size()");
   bad += !throws([&] { p.setDensity(std::vector<double>{}, {}, {}); }, "must not be empty");
+  bad += !throws([&] { p.setDensity({0.0, 1.0}, {1.0}, {0.0, 0.0}, {1.0, 1.0}, {0.5, 0.5}); },
+                 "a.size() != radii.size()");
   bad += !throws([&] { p.setDensityFromFile("/nonexistent/model.dat"); }, "could not open");
   p.setCosineList({-1.0, -0.5, 0.2});
   p.setEnergyList({1.0, 2.0, 3.0, 4.0});
@@ -145,12 +159,13 @@ int main(int argc, char** argv) {
   try {
     if (argc >= 2 && std::string(argv[1]) == "errors") return run_errors();
     if (argc < 5) {
-      std::fprintf(stderr, "usage: %s atmos|beam out.bin precision nutype | errors\n", argv[0]);
+      std::fprintf(stderr, "usage: %s atmos|poly|beam out.bin precision nutype | errors\n", argv[0]);
       return 2;
     }
     const std::string mode = argv[1];
     const int prec = std::atoi(argv[3]), nt = std::atoi(argv[4]);
     if (mode == "atmos") return run_atmos(argv[2], prec, nt);
+    if (mode == "poly") return run_atmos(argv[2], prec, nt, true);
     if (mode == "beam") return run_beam(argv[2], prec, nt);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "exception: %s\n", e.what());

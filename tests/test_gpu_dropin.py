@@ -50,6 +50,19 @@ def test_dropin_atmos_mixed(oracle, tmp_path):
 
 
 @pytest.mark.parametrize("nutype", [0, 1])
+def test_dropin_atmos_poly_fp64(oracle, tmp_path, nutype):
+    """setDensity(radii, a, b, c, yps) through the C++ class."""
+    out = tmp_path / "p.bin"
+    assert run("poly", out, 0, nutype).returncode == 0
+    got = np.fromfile(out, dtype=np.float64).reshape(100, 100, 9)
+    ep = W.prem4_poly()
+    ref = oracle.atmos_poly(nutype, W.cos_grid(100), W.log_grid(100, 0.1, 100.0),
+                            ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                            osc_for(NOM, nutype), 22.0, threads=4)
+    assert np.abs(got - ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
 def test_dropin_beam_fp64(oracle, tmp_path, nutype):
     out = tmp_path / "b.bin"
     assert run("beam", out, 0, nutype).returncode == 0

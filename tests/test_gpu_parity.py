@@ -180,3 +180,83 @@ def test_mixed_error_shells59_full_grid(oracle, nutype):
         print(f"59 shells 400x400 nutype {nutype}: mixed max {d.max():.2e} "
               f"rms {np.sqrt((d**2).mean()):.2e}")
         assert d.max() < TOL_MIXED
+
+
+# ------------------------------------------------------ polynomial densities
+
+def run_poly(cz, ev, ep, params, nutype, precision="fp64", path=None, nc=0, w=0,
+             mode="auto"):
+    from cudaprob3_amd import AtmosPropagator
+    with AtmosPropagator(len(cz), len(ev), precision=precision) as p:
+        p.setSolveMode(mode)
+        if nc or w:
+            p.setTuning(nc, w)
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        t12, t13, t23, d, m12, m23 = [float(x) for x in params]
+        p.setMNSMatrix(t12, t13, t23, d, -1 if nutype else 1)
+        p.setNeutrinoMasses(m12, m23)
+        if path:
+            p.setDensityFromFile(path)
+        else:
+            p.setDensity(ep.radii, ep.a, ep.b, ep.c, ep.yps)
+        p.setProductionHeight(22.0)
+        p.calculateProbabilities(nutype)
+        return p.probabilities()
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_poly_fp64_vs_golden(nutype):
+    """setDensity(radii, a, b, c, yps): the compiled reference's polynomial
+    path (golden g6, cos = -1 included) within 1e-10, unitarity kept."""
+    g = golden(f"g6_prem4poly_{'nubar' if nutype else 'nu'}")
+    ep = W.prem4_poly()
+    out = run_poly(g["cosv"], g["ev"], ep, NOM, nutype)
+    print("poly nutype", nutype, "max|d| vs golden", np.abs(out - g["prob"]).max())
+    assert np.abs(out - g["prob"]).max() < TOL_FP64
+    assert unitarity(out) < 1e-12
+
+
+def test_poly_against_oracle_file_and_tunings(oracle, tmp_path):
+    """A 200 x 200 grid through the polynomial model against the oracle:
+    FP64 within 1e-10, mixed within 1e-5, FP32 within 2e-3; the 5-column
+    file gives the same bits as the arrays; tunings and the solve mode do
+    not change a bit (the model solves per trajectory)."""
+    ep = W.prem4_poly()
+    cz, ev = W.cos_grid(200), W.log_grid(200, 0.1, 100.0)
+    path = str(tmp_path / "poly.dat")
+    ep.write(path)
+    for nt in (0, 1):
+        ref = oracle.atmos_poly(nt, cz, ev, ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                                osc_for(NOM, nt), 22.0, threads=8)
+        f64 = run_poly(cz, ev, ep, NOM, nt)
+        mixed = run_poly(cz, ev, ep, NOM, nt, "mixed")
+        f32 = run_poly(cz, ev, ep, NOM, nt, "fp32")
+        print("poly 200x200 nutype", nt, "fp64", np.abs(f64 - ref).max(),
+              "mixed", np.abs(mixed - ref).max(), "fp32", np.abs(f32 - ref).max())
+        assert np.abs(f64 - ref).max() < TOL_FP64
+        assert np.abs(mixed - ref).max() < TOL_MIXED
+        assert np.abs(f32 - ref).max() < TOL_FP32
+        assert np.array_equal(run_poly(cz, ev, ep, NOM, nt, path=path), f64)
+        assert np.array_equal(run_poly(cz, ev, ep, NOM, nt, "mixed", nc=4, w=4,
+                                       mode="table"), mixed)
+
+
+def test_poly_then_constant_model():
+    """A constant model set after a polynomial one is used as such."""
+    from cudaprob3_amd import AtmosPropagator
+    e, ep = W.prem4(), W.prem4_poly()
+    cz, ev = W.cos_grid(30), W.log_grid(50, 0.1, 100.0)
+    base = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0)
+    with AtmosPropagator(len(cz), len(ev)) as p:
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        t12, t13, t23, d, m12, m23 = NOM
+        p.setMNSMatrix(t12, t13, t23, d, 1)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(ep.radii, ep.a, ep.b, ep.c, ep.yps)
+        p.setProductionHeight(22.0)
+        p.calculateProbabilities(0)
+        p.setDensity(e.radii, e.rhos, e.yps)
+        p.calculateProbabilities(0)
+        assert np.array_equal(p.probabilities(), base)

@@ -22,6 +22,12 @@ def model():
                               dp, dp, C.c_int] + [C.c_double] * 4 +
                              [C.c_int] + [C.c_double] * 3 +
                              [dp, C.c_char_p, C.c_int])
+    m.cp3m_atmos_poly.argtypes = ([C.c_int, C.c_int, dp, C.c_int, dp, C.c_int] +
+                                  [dp] * 5 + [C.c_int] + [C.c_double] * 4 +
+                                  [C.c_int] + [C.c_double] * 3 +
+                                  [dp, C.c_char_p, C.c_int])
+    m.cp3m_poly_rye.argtypes = ([dp, C.c_int] + [dp] * 5 +
+                                [C.c_int, C.c_double, dp])
     m.cp3m_beam.argtypes = ([C.c_int, C.c_int, dp, C.c_int, C.c_double,
                              C.c_double] + [C.c_double] * 4 + [C.c_int] +
                             [C.c_double] * 2 + [dp, C.c_char_p, C.c_int])
@@ -146,3 +152,46 @@ def test_phase_polynomial():
         t = f[n + 1:]
         assert np.all(np.abs(wi[n + 1:] / (2 * np.pi * t) - 1) < 1e-6)
         assert np.all(np.abs(wr[n + 1:] / (-2 * (np.pi * t) ** 2) - 1) < 1e-5)
+
+
+def m_atmos_poly(m, prec, nt, cz, ev, ep, row, h=22.0):
+    cz = np.ascontiguousarray(cz, float)
+    ev = np.ascontiguousarray(ev, float)
+    out = np.zeros((len(cz), len(ev), 9))
+    err = C.create_string_buffer(256)
+    t12, t13, t23, d, m12, m23 = [float(x) for x in row]
+    rc = m.cp3m_atmos_poly(prec, nt, P(cz), len(cz), P(ev), len(ev),
+                           P(ep.radii), P(ep.a), P(ep.b), P(ep.c), P(ep.yps),
+                           len(ep.radii), t12, t13, t23, d, -1 if nt else 1,
+                           m12, m23, h, out.ctypes.data_as(dp), err, 256)
+    assert rc == 0, err.value
+    return out
+
+
+def test_poly_trajectory_densities_match_oracle(model, oracle):
+    """The product's per-(cosine, layer) densities of a polynomial model are
+    the oracle's (pinned to the compiled reference) bit for bit."""
+    ep = W.prem4_poly()
+    cz = np.concatenate([W.cos_grid(64), [-0.999999, -1e-6]])
+    out = np.zeros((len(cz), len(ep.radii)))
+    assert model.cp3m_poly_rye(P(cz), len(cz), P(ep.radii), P(ep.a), P(ep.b),
+                               P(ep.c), P(ep.yps), len(ep.radii), 22.0,
+                               out.ctypes.data_as(dp)) == 0
+    ref = oracle.poly_rye(cz, ep.radii, ep.a, ep.b, ep.c, ep.yps, 22.0)
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_poly_rewrite_vs_golden_and_oracle(model, oracle, nutype):
+    """The kernel arithmetic through a polynomial model: FP64 within 1e-10 of
+    the compiled reference's golden and of the oracle, mixed within 1e-5."""
+    g = golden(f"g6_prem4poly_{'nubar' if nutype else 'nu'}")
+    ep = W.prem4_poly()
+    assert np.array_equal(g["a"], ep.a) and np.array_equal(g["c"], ep.c)
+    out = m_atmos_poly(model, 0, nutype, g["cosv"], g["ev"], ep, NOM)
+    assert np.abs(out - g["prob"]).max() < 1e-10
+    cz, ev = W.cos_grid(60), W.log_grid(60, 0.1, 100.0)
+    ref = oracle.atmos_poly(nutype, cz, ev, ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                            osc_for(NOM, nutype), 22.0, threads=4)
+    assert np.abs(m_atmos_poly(model, 0, nutype, cz, ev, ep, NOM) - ref).max() < 1e-10
+    assert np.abs(m_atmos_poly(model, 2, nutype, cz, ev, ep, NOM) - ref).max() < 1e-5
