@@ -179,7 +179,7 @@ CP3_HD void m_sincos(float x, float* s, float* c) {
 
 // w = e^{2 pi i f} - 1 for |f| <= 1/2, with no argument reduction and no
 // quadrant: a = sqrt(2) sin(pi f) and b = sqrt(2) cos(pi f) from minimax
-// polynomials in f (degrees 9 and 10 in FP32, 17 and 16 in FP64; the
+// polynomials in f (degrees 9 and 10 in FP32, 15 and 16 in FP64; the
 // coefficients and their errors, at most 2.1e-7 and 4.5e-16 with rounding,
 // come from fit_cispi.py), then w = (cos 2 pi f - 1, sin 2 pi f)
 // = (-2 sin^2, 2 sin cos)(pi f) = (-a^2, a b), free of cancellation for
@@ -203,15 +203,14 @@ CP3_HD void m_cispi_m1(float f, float* wr, float* wi) {
 }
 CP3_HD void m_cispi_m1(double f, double* wr, double* wi) {
   const double z = f * f;
-  double pa = 1.0866733025717579e-06;
-  pa = fma(pa, z, -3.097413590743419e-05);
-  pa = fma(pa, z, 0.0006594466111570788);
-  pa = fma(pa, z, -0.01042336257001818);
-  pa = fma(pa, z, 0.11617182685681435);
-  pa = fma(pa, z, -0.8474880248098783);
-  pa = fma(pa, z, 3.606476571470557);
-  pa = fma(pa, z, -7.308249499995437);
-  pa = fma(pa, z, 4.442882938158366);
+  double pa = -2.9819668671076778e-05;
+  pa = fma(pa, z, 0.0006589415933150775);
+  pa = fma(pa, z, -0.010423245350225985);
+  pa = fma(pa, z, 0.11617181136154904);
+  pa = fma(pa, z, -0.8474880236479588);
+  pa = fma(pa, z, 3.606476571424355);
+  pa = fma(pa, z, -7.308249499994612);
+  pa = fma(pa, z, 4.442882938158362);
   const double a = pa * f;
   double b = 5.8684089087724815e-06;
   b = fma(b, z, -0.00014787947669297268);

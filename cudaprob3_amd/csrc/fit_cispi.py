@@ -93,5 +93,5 @@ if __name__ == "__main__":
     cos = lambda x: mp.sqrt(2) * mp.cos(mp.pi * x)
     report("sqrt(2) sin(pi f)", sin, 4, True, np.float32)
     report("sqrt(2) cos(pi f)", cos, 5, False, np.float32)
-    report("sqrt(2) sin(pi f)", sin, 8, True, np.float64)
+    report("sqrt(2) sin(pi f)", sin, 7, True, np.float64)
     report("sqrt(2) cos(pi f)", cos, 8, False, np.float64)
