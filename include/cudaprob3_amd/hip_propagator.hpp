@@ -192,7 +192,18 @@ class HipPropagatorSingle {
   void calculateProbabilitiesAsync(NeutrinoType type) {
     check(cp3_atmos_calculate_async(live(), type == Antineutrino ? 1 : 0));
   }
+  // atmoscudapropagator.cuh:114-120 (the CUDA class's names)
+  void calculateAtmosProbabilitiesAsync(NeutrinoType type) { calculateProbabilitiesAsync(type); }
   void waitForCompletion() { check(cp3_atmos_wait(live())); }
+  // atmoscudapropagator.cu:319-325: waits, then copies the whole result to
+  // host memory (the first probability read does exactly that).
+  void getResultFromDevice() {
+    double v = 0;
+    check(cp3_atmos_get_probability(live(), 0, 0, 0, &v));
+  }
+  // atmoscpupropagator.hpp:431-444: the shell counts are rebuilt with the
+  // trajectory table whenever the cosines or the model change.
+  void setMaxlayers() {}
 
   // atmoscudapropagator.cu:222-235
   FLOAT_T getProbability(int index_cosine, int index_energy, ProbType t) {
@@ -288,7 +299,13 @@ class HipBeamPropagatorSingle {
   void calculateProbabilitiesAsync(NeutrinoType type) {
     check(cp3_beam_calculate_async(live(), type == Antineutrino ? 1 : 0));
   }
+  // beamcudapropagator.cuh:95-101 (the CUDA class's names)
+  void calculateBeamProbabilitiesAsync(NeutrinoType type) { calculateProbabilitiesAsync(type); }
   void waitForCompletion() { check(cp3_beam_wait(live())); }
+  void getResultFromDevice() {
+    double v = 0;
+    check(cp3_beam_get_probability(live(), 0, 0, &v));
+  }
   // beamcudapropagator.cu:146-159
   FLOAT_T getProbability(int index_energy, ProbType t) {
     double v = 0;

@@ -58,7 +58,15 @@ static int run_atmos(const char* path, int precision, int nutype, bool poly = fa
                  {0.5, 0.5, 0.5, 0.5, 0.5});
   }
   p.setProductionHeight(22.0);
-  p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
+  p.setMaxlayers();
+  if (poly) {
+    p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
+  } else {
+    // the CUDA class's asynchronous names (atmoscudapropagator.cuh:114-120)
+    p.calculateAtmosProbabilitiesAsync(nutype ? Antineutrino : Neutrino);
+    p.waitForCompletion();
+    p.getResultFromDevice();
+  }
   std::vector<double> out((size_t)nC * nE * 9);
   for (int c = 0; c < nC; ++c)
     for (int e = 0; e < nE; ++e)
@@ -90,7 +98,9 @@ static int run_beam(const char* path, int precision, int nutype) {
   p.setNeutrinoMasses(kDm12, kDm23);
   p.setDensity(2.6);
   p.setPathLength(1300.0);
-  p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
+  p.calculateBeamProbabilitiesAsync(nutype ? Antineutrino : Neutrino);
+  p.waitForCompletion();
+  p.getResultFromDevice();
   std::vector<double> out((size_t)nE * 9);
   for (int e = 0; e < nE; ++e)
     for (int t = 0; t < 9; ++t) out[(size_t)e * 9 + t] = p.getProbability(e, (ProbType)t);
