@@ -290,6 +290,13 @@ class HipBeamPropagatorSingle {
   }
   // beamcpupropagator.hpp:84-91
   void setDensity(FLOAT_T rho) { check(cp3_beam_set_density(live(), rho)); }
+  // beamcpupropagator.hpp:112-120: a beam holds no per-row composition (its
+  // Ye is 0.5, physics.hpp:1219), so only an empty list is accepted.
+  void setChemicalComposition(const std::vector<FLOAT_T>& list) {
+    if (!list.empty())
+      throw std::runtime_error(
+          "BeamCpuPropagator::setChemicalComposition. Size of input list not equal to expectation");
+  }
   void setPathLength(FLOAT_T path_length) {
     check(cp3_beam_set_path_length(live(), path_length));
   }

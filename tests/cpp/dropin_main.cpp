@@ -161,6 +161,9 @@ static int run_errors() {
     ++bad;
   }
   bad += !throws([] { HipPropagatorSingle<double> r(1 << 20, 2, 2); }, "not available");
+  HipBeamPropagatorSingle<double> beam(0, 4, 1);
+  beam.setChemicalComposition({});
+  bad += !throws([&] { beam.setChemicalComposition({0.5}); }, "Size of input list");
   std::printf("errors: %d unexpected\n", bad);
   return bad ? 1 : 0;
 }
