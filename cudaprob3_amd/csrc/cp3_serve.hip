@@ -15,6 +15,7 @@
 // Every tuning of the kernels gives bitwise identical results, so a request's
 // probabilities do not depend on which other requests share its batch.
 #include <hip/hip_runtime.h>
+#include <sys/prctl.h>
 
 #include <algorithm>
 #include <chrono>
@@ -302,9 +303,22 @@ struct cp3_server {
     s.busy = false;
   }
 
+  // Poll interval while batches are in flight, and how long the scheduler
+  // keeps polling after its last launch or completion before it blocks on
+  // the submit signal (a blocked thread takes tens of microseconds to wake;
+  // the polling thread sees a submission within kPollUs).
+  static constexpr int kPollUs = 2;
+  static constexpr int kIdleSpinUs = 1000;
+
   void loop() {
     DeviceScope ds(dev);
+    // The poll below sleeps kPollUs between event queries while batches are
+    // in flight; the default 50 us timer slack of a Linux thread would
+    // stretch each sleep to about 60 us, and a request's completion (or a new
+    // submission) is noticed only at the next poll.
+    (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
     std::unique_lock<std::mutex> lk(mu);
+    Clock::time_point last = Clock::now();
     for (;;) {
       bool progress = false, busy = false;
       for (auto& sp : slots) {
@@ -328,10 +342,13 @@ struct cp3_server {
         progress = true;
       }
       if (stop && !busy) break;
-      if (progress) continue;
-      if (busy) {
+      if (progress) {
+        last = Clock::now();
+        continue;
+      }
+      if (busy || Clock::now() - last < std::chrono::microseconds(kIdleSpinUs)) {
         lk.unlock();
-        std::this_thread::sleep_for(std::chrono::microseconds(10));
+        std::this_thread::sleep_for(std::chrono::microseconds(kPollUs));
         lk.lock();
       } else {
         cv_work.wait(lk);
