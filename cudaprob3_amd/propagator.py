@@ -229,6 +229,12 @@ class BeamPropagator:
     def calculateProbabilities(self, nutype=Neutrino):
         N.check(self._lib.cp3_beam_calculate(self._h, int(nutype)))
 
+    def calculateProbabilitiesAsync(self, nutype=Neutrino):
+        N.check(self._lib.cp3_beam_calculate_async(self._h, int(nutype)))
+
+    def waitForCompletion(self):
+        N.check(self._lib.cp3_beam_wait(self._h))
+
     def getProbability(self, index_energy, t) -> float:
         v = C.c_double()
         N.check(self._lib.cp3_beam_get_probability(
