@@ -9,7 +9,11 @@ the whole batch with every input resident in HBM.  The mixed form (FP64
 eigen-solve and phases, FP32 amplitudes) is the headline; the plain FP64 form
 of the same batch is timed beside it.
 
-Prints one JSON line (see DESIGN.md for the fields).
+Prints one JSON line: the headline metric and step time, the per-phase
+kernel times, the dominant kernel's roofline (counted FP64-equivalent FLOPs
+over its own launch time, and its HBM traffic from the committed rocprofv3
+counters), the plain FP64 form, the serving workload's latency percentiles,
+and the reference CPU propagator timed on a sample of the same batch.
 """
 from __future__ import annotations
 
@@ -176,8 +180,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the propagator shards by parameter set with no exchange; "
-              "see DESIGN.md)", file=sys.stderr)
+              "(several GPUs would each take a share of the parameter sets, "
+              "with no exchange between them)", file=sys.stderr)
         sys.exit(2)
 
     if args.calib_copy_mib:
