@@ -260,3 +260,30 @@ def test_poly_then_constant_model():
         p.setDensity(e.radii, e.rhos, e.yps)
         p.calculateProbabilities(0)
         assert np.array_equal(p.probabilities(), base)
+
+
+def test_wide_parameter_sweep(oracle):
+    """24 seeded parameter sets from wide ranges (either mass ordering,
+    theta13 up to 0.5, delta over the circle) on the 12-shell model: FP64
+    within 1e-10 of the oracle, mixed within 1e-5, and FP64 rows and columns
+    of P summing to one within 1e-11 (the compiled reference itself reaches
+    7.5e-13 on one of these sets: FP64 rounding over the layers)."""
+    rng = np.random.default_rng(20261203)
+    e = W.earth_12()
+    cz, ev = W.cos_grid(48), W.log_grid(48, 0.1, 100.0)
+    worst64, worstm = 0.0, 0.0
+    for k in range(24):
+        par = [rng.uniform(0.1, 1.4), rng.uniform(0.0, 0.5), rng.uniform(0.5, 1.1),
+               rng.uniform(-np.pi, np.pi), rng.uniform(1e-5, 2e-4),
+               rng.choice([-1.0, 1.0]) * rng.uniform(1e-3, 4e-3)]
+        nt = k % 2
+        ref = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(par, nt),
+                           22.0, threads=8)
+        f64 = run_atmos(cz, ev, e.radii, e.rhos, e.yps, par, nt)
+        mixed = run_atmos(cz, ev, e.radii, e.rhos, e.yps, par, nt, "mixed")
+        worst64 = max(worst64, float(np.abs(f64 - ref).max()))
+        worstm = max(worstm, float(np.abs(mixed - ref).max()))
+        assert unitarity(f64) < 1e-11, par
+    print("wide sweep: fp64", worst64, "mixed", worstm)
+    assert worst64 < TOL_FP64
+    assert worstm < TOL_MIXED
