@@ -36,6 +36,8 @@ METRIC = "prob evals/s (grid pts x param sets), p50/p99 latency; mixed vs plain 
 PEAK_F64 = 78.6e12   # MI355X FP64 vector/matrix, FLOP/s
 PEAK_F32 = 157.3e12  # MI355X FP32 vector (published, packed), FLOP/s
 PEAK_HBM = 8.0e12    # B/s
+N_CU = 256           # MI355X compute units
+CLOCK_HZ = 2.4e9     # MI355X peak engine clock
 
 
 def build_batch(precision, wl, tuning=None, graph=True):
@@ -211,6 +213,17 @@ def main():
     dom = kernel_name("mixed", tune_m)
     traffic_rec = FL.measured_traffic(dom)
     traffic = traffic_rec["bytes"] if traffic_rec else None
+    # the issue view: the kernel's VALU instructions (rocprofv3 SQ_INSTS_VALU,
+    # same tuning and workload) over its launch time in this run, against the
+    # rate an FMA-chain probe reaches at the kernel's occupancy
+    valu_rec = FL.measured_valu(dom)
+    valu = None
+    if valu_rec:
+        rate = valu_rec["sq_insts_valu_per_launch"] / t_dom / (N_CU * CLOCK_HZ)
+        cap = valu_rec["probe_fp32_fma_wave_instr_per_cu_cycle"]["4_waves_per_simd"]
+        valu = {"wave_instr_per_cu_cycle": rate, "probe_at_same_occupancy": cap,
+                "frac": rate / cap, "clock_hz_assumed": CLOCK_HZ,
+                "detail": valu_rec}
 
     result = {
         "metric": METRIC,
@@ -249,6 +262,7 @@ def main():
             "frac": achieved / PEAK_F64,
             "traffic": traffic,
             "traffic_detail": traffic_rec,
+            "valu_issue": valu,
             "kernel_ms": phases["atmos"],
             "note": "FP64-equivalent rate (F64 + F32*78.6/157.3) of the "
                     "counted FLOPs of the exact kernel decomposition over the "

@@ -148,6 +148,18 @@ def beam_counts(n_points: int, precision: str) -> dict:
     return {k: n_points * ops["beam"][k] for k in KEYS}
 
 
+def measured_valu(name: str):
+    """VALU wave-instructions per launch measured with rocprofv3 counters
+    (SQ_INSTS_VALU) and the issue rates of an FMA-chain probe, as recorded in
+    profiles/valu_counters.json, or None."""
+    path = os.path.join(PROFILES, "valu_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data.get(name)
+
+
 def measured_traffic(name: str):
     """HBM bytes per launch measured with rocprofv3 counters (FETCH_SIZE,
     WRITE_SIZE), as recorded in profiles/traffic.json, or None."""
