@@ -14,6 +14,8 @@
 // [9][nC][nE] output.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cp3_launch.hpp"
 #include "cp3_math.hpp"
 
@@ -554,21 +556,23 @@ __global__ void __launch_bounds__(kWave) beam_single_kernel(BeamJob job) {
   beam_unit<P>(job, (int64_t)blockIdx.x);
 }
 
+// One work unit (64 energies of one job) per workgroup: without a loop the
+// solve's constants are not held in registers across units, which keeps the
+// kernel at about 50 VGPRs (8 waves per SIMD instead of 4).
 template <class P>
 __global__ void __launch_bounds__(kWave) beam_batch_kernel(
-    const BeamJob* __restrict__ jobs, int njobs, int64_t nunits) {
-  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].unit_begin <= u)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    const BeamJob& job = jobs[lo];
-    beam_unit<P>(job, u - job.unit_begin);
+    const BeamJob* __restrict__ jobs, int njobs, int64_t unit0) {
+  const int64_t u = unit0 + blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].unit_begin <= u)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
+  const BeamJob& job = jobs[lo];
+  beam_unit<P>(job, u - job.unit_begin);
 }
 
 // ------------------------------------------------------------- launchers
@@ -783,14 +787,22 @@ hipError_t launch_beam_batch(int precision, const BeamJob* jobs, int njobs,
                              int64_t nunits, int grid, hipStream_t st) {
   if (nunits <= 0 || njobs <= 0) return hipSuccess;
   if (grid <= 0) return hipErrorInvalidConfiguration;
-  const dim3 g((unsigned)grid), b(kWave);
-  switch (precision) {
-    case kFP64: beam_batch_kernel<PolicyF64><<<g, b, 0, st>>>(jobs, njobs, nunits); break;
-    case kFP32: beam_batch_kernel<PolicyF32><<<g, b, 0, st>>>(jobs, njobs, nunits); break;
-    case kMixed: beam_batch_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs, njobs, nunits); break;
-    default: return hipErrorInvalidValue;
+  // one workgroup per unit, in launches of at most 2^31 - 1 workgroups (the
+  // grid argument, a persistent grid size, is not used by this kernel)
+  (void)grid;
+  const int64_t kMaxGrid = 0x7fffffff;
+  for (int64_t u0 = 0; u0 < nunits; u0 += kMaxGrid) {
+    const dim3 g((unsigned)std::min<int64_t>(nunits - u0, kMaxGrid)), b(kWave);
+    switch (precision) {
+      case kFP64: beam_batch_kernel<PolicyF64><<<g, b, 0, st>>>(jobs, njobs, u0); break;
+      case kFP32: beam_batch_kernel<PolicyF32><<<g, b, 0, st>>>(jobs, njobs, u0); break;
+      case kMixed: beam_batch_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs, njobs, u0); break;
+      default: return hipErrorInvalidValue;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 template <class P>
