@@ -128,6 +128,8 @@ hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* d_job
                              hipStream_t stream);
 hipError_t launch_beam_single(int precision, const BeamJob& job,
                               hipStream_t stream);
+// The batched beam kernel runs one workgroup per unit whatever `grid` asks
+// (a persistent loop would hold the solve's constants in VGPRs).
 hipError_t launch_beam_batch(int precision, const BeamJob* d_jobs, int njobs,
                              int64_t nunits, int grid, hipStream_t stream);
 
