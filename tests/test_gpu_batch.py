@@ -250,3 +250,33 @@ def test_phase_timings(mode):
         else:  # nothing launched: only the events' own spacing
             assert ph["prepass"] < 0.05
     b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("mode", ["shared", "table"])
+def test_random_grid_shapes(precision, mode):
+    """Seeded random grids: 1-70 cosines drawn over [-1, 1) (down-going
+    trajectories without shells among them), 1-200 energies, several jobs per
+    grid so that packed tails form; every result equals the single
+    propagator's bitwise."""
+    rng = np.random.default_rng(777)
+    earth = W.earth_59()
+    sets = W.draw_param_sets(3, 778)
+    grids = {}
+    for k in range(6):
+        nc, ne = int(rng.integers(1, 71)), int(rng.integers(1, 201))
+        cz = np.sort(rng.uniform(-1.0, 1.0, nc))
+        ev = np.sort(rng.uniform(0.2, 60.0, ne))
+        grids[f"r{k}"] = (cz, ev)
+    plan = [(g, s, nt) for g in grids for s in sets for nt in (0, 1)]
+    b = Batch(precision)
+    b.set_solve_mode(mode)
+    gid = {k: b.add_atmos_grid(c, e, earth, 22.0) for k, (c, e) in grids.items()}
+    jobs = [W.AtmosJob(p, grids[g][0], grids[g][1], nt, grid=g) for g, p, nt in plan]
+    b.set_jobs([make_job(ATMOS, gid[j.grid], j.params, j.nutype) for j in jobs])
+    b.run()
+    b.wait()
+    for i, job in enumerate(jobs):
+        ref = single_atmos(job, earth, precision)
+        assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
+    b.close()
