@@ -183,10 +183,9 @@ struct cp3_atmos {
       job.slabs = d_slabs.p;
       d_job.alloc(sizeof(job), dev);
       cp3::AtmosJob* dj = (cp3::AtmosJob*)d_job.p;
-      hip_check(cp3::launch_store_job(job, dj, stream), "job descriptor store");
       const int64_t units = cp3::cosine_blocks(job.g, nc, 1) * job.etiles;
       hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
-      hip_check(cp3::launch_slab_table(precision, dj, 1, cp3::slab_waves(job.g), stream),
+      hip_check(cp3::launch_slab_table_single(precision, job, dj, stream),
                 "slab table launch");
       hip_check(cp3::launch_atmos_table(precision, nc, dj, 1, units,
                                         (int)((units + 7) / 8 * 8), stream),

@@ -388,30 +388,17 @@ __global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_tai
 // table among every cosine of the grid).  Table of a job: Slab<P> rows
 // [g.S + 1][energy_tiles(g) * 64], row g.S the vacuum.
 
-// Writes one job descriptor, passed as a kernel argument, to device memory in
-// stream order (no host staging buffer to keep alive until the copy runs).
-__global__ void __launch_bounds__(kWave) store_job_kernel(AtmosJob job, AtmosJob* dst) {
-  static_assert(sizeof(AtmosJob) % 4 == 0, "AtmosJob is copied as 32-bit words");
-  constexpr int kWords = (int)(sizeof(AtmosJob) / 4);
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(&job);
-  uint32_t* out = reinterpret_cast<uint32_t*>(dst);
-  for (int i = (int)threadIdx.x; i < kWords; i += kWave) out[i] = src[i];
-}
-
 // Pre-pass: the (S + 1) x nE used entries of each job's table, 64
 // consecutive (row, energy) pairs per wave, row-major, so that no lane solves
-// a padding column; on a 2-D grid (y: job, x: the job's waves) so that a wave
-// finds its job without a search; waves past a smaller job's count exit at
-// once.
+// a padding column.  Wave w of job `job`:
 template <class P>
-__global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __restrict__ jobs) {
+__device__ __forceinline__ void slab_wave(const AtmosJob& job, int64_t w) {
   using D = typename P::D;
   const int lane = (int)threadIdx.x;
-  const AtmosJob& job = jobs[blockIdx.y];
   const GridDev& g = job.g;
-  if ((int64_t)blockIdx.x >= slab_waves(g)) return;
+  if (w >= slab_waves(g)) return;
   // the wave's first pair (row i0, energy e0), wave-uniform
-  const int64_t first = (int64_t)blockIdx.x * kWave;
+  const int64_t first = w * kWave;
   const int i0 = (int)(first / g.nE);
   int i = i0;
   int e = (int)(first - (int64_t)i0 * g.nE) + lane;
@@ -426,6 +413,30 @@ __global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __res
   Slab<P> sl;
   solve_slab<P>(job.sc, E, D(1) / E, i < g.S ? rye : D(0), sl);
   ((Slab<P>*)job.slabs)[(int64_t)i * energy_tiles(g) * kWave + e] = sl;
+}
+
+// The batch's pre-pass on a 2-D grid (y: job, x: the job's waves) so that a
+// wave finds its job without a search; waves past a smaller job's count exit
+// at once.
+template <class P>
+__global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __restrict__ jobs) {
+  slab_wave<P>(jobs[blockIdx.y], (int64_t)blockIdx.x);
+}
+
+// The single grid's pre-pass, its job passed as a kernel argument; workgroup
+// 0 also stores the job descriptor for the propagation kernel that follows
+// in stream order (no host staging buffer and no extra launch).
+template <class P>
+__global__ void __launch_bounds__(kWave) slab_table_single_kernel(AtmosJob job,
+                                                                  AtmosJob* dst) {
+  static_assert(sizeof(AtmosJob) % 4 == 0, "AtmosJob is copied as 32-bit words");
+  if (blockIdx.x == 0) {
+    constexpr int kWords = (int)(sizeof(AtmosJob) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&job);
+    uint32_t* out = reinterpret_cast<uint32_t*>(dst);
+    for (int i = (int)threadIdx.x; i < kWords; i += kWave) out[i] = src[i];
+  }
+  slab_wave<P>(job, (int64_t)blockIdx.x);
 }
 
 // The NC trajectories of cosine block sb (NC cosines) for one energy per
@@ -676,8 +687,17 @@ size_t slab_entry_bytes(int precision) {
   }
 }
 
-hipError_t launch_store_job(const AtmosJob& job, AtmosJob* dst, hipStream_t st) {
-  store_job_kernel<<<dim3(1), dim3(kWave), 0, st>>>(job, dst);
+hipError_t launch_slab_table_single(int precision, const AtmosJob& job, AtmosJob* d_job,
+                                   hipStream_t st) {
+  const int64_t waves = slab_waves(job.g);
+  if (waves <= 0 || waves > 0x7fffffff) return hipErrorInvalidConfiguration;
+  const dim3 g((unsigned)waves), b(kWave);
+  switch (precision) {
+    case kFP64: slab_table_single_kernel<PolicyF64><<<g, b, 0, st>>>(job, d_job); break;
+    case kFP32: slab_table_single_kernel<PolicyF32><<<g, b, 0, st>>>(job, d_job); break;
+    case kMixed: slab_table_single_kernel<PolicyMixed><<<g, b, 0, st>>>(job, d_job); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

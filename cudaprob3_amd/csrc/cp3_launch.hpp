@@ -111,8 +111,10 @@ hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* d_jo
 // slab_waves; at most 65535 jobs), and the propagation launches (one wave
 // per workgroup) reading the tables.
 size_t slab_entry_bytes(int precision);
-// Copies one job descriptor to device memory in stream order.
-hipError_t launch_store_job(const AtmosJob& job, AtmosJob* d_job, hipStream_t stream);
+// The single grid's pre-pass (job by value), which also stores the job
+// descriptor at d_job for the propagation launch that follows.
+hipError_t launch_slab_table_single(int precision, const AtmosJob& job, AtmosJob* d_job,
+                                   hipStream_t stream);
 hipError_t launch_slab_table(int precision, const AtmosJob* d_jobs, int njobs,
                              int64_t max_waves, hipStream_t stream);
 hipError_t launch_atmos_table(int precision, int nc, const AtmosJob* d_jobs, int njobs,
