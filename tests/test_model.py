@@ -195,3 +195,26 @@ def test_poly_rewrite_vs_golden_and_oracle(model, oracle, nutype):
                             osc_for(NOM, nutype), 22.0, threads=4)
     assert np.abs(m_atmos_poly(model, 0, nutype, cz, ev, ep, NOM) - ref).max() < 1e-10
     assert np.abs(m_atmos_poly(model, 2, nutype, cz, ev, ep, NOM) - ref).max() < 1e-5
+
+
+def test_fp32_form_vs_reference_float_build(model, oracle):
+    """The FP32 form is at least as accurate as the reference's own
+    AtmosCpuPropagator<float> against FP64 (where oracle/_ref is built); the
+    mixed form is far inside both."""
+    from oracle.loader import load_ref
+    ref = load_ref(deep=True)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    e = W.earth_12()
+    cz, ev = W.cos_grid(80), W.log_grid(80, 0.1, 100.0)
+    for nt in (0, 1):
+        exact = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(NOM, nt),
+                             22.0, threads=4)
+        ref32 = np.abs(ref.atmos(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(NOM, nt),
+                                 22.0, threads=4, f32=True) - exact).max()
+        ours32 = np.abs(m_atmos(model, 1, nt, cz, ev, e, NOM) - exact).max()
+        mixed = np.abs(m_atmos(model, 2, nt, cz, ev, e, NOM) - exact).max()
+        print(f"nutype {nt}: reference float {ref32:.2e}, FP32 form {ours32:.2e}, "
+              f"mixed {mixed:.2e}")
+        assert ours32 <= ref32
+        assert mixed < 1e-5
