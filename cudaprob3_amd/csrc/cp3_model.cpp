@@ -238,6 +238,34 @@ void cp3m_cispi_m1(int precision, const double* f, int n, double* wr,
   }
 }
 
+// A density file as the product loads it (setDensityFromFile): rows stored
+// descending, 5 values per row (radius, rho or a, b, c, Ye; b = c = 0 for a
+// constant model); *poly = 1 for a polynomial model.  Returns -1 with the
+// message in err on an unsupported or unreadable file, -2 if cap is short.
+int cp3m_load_earth_file(const char* path, double* rows, int cap, int* nrows, int* poly,
+                         char* err, int errlen) {
+  try {
+    cp3::EarthModel em;
+    cp3::earth_load_file(em, path);
+    const size_t n = em.radii.size();
+    if ((int)n > cap) return -2;
+    for (size_t i = 0; i < n; ++i) {
+      double* r = rows + 5 * i;
+      r[0] = em.radii[i];
+      r[1] = em.poly ? em.as[i] : em.rhos[i];
+      r[2] = em.poly ? em.bs[i] : 0.0;
+      r[3] = em.poly ? em.cs[i] : 0.0;
+      r[4] = em.yps[i];
+    }
+    *nrows = (int)n;
+    *poly = em.poly ? 1 : 0;
+  } catch (const std::exception& ex) {
+    put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
 // Shell counts per cosine (the M of the trajectory table) for a model.
 int cp3m_max_layers(const double* cosv, int nC, const double* radii,
                     const double* rhos, const double* yps, int nrows,

@@ -218,3 +218,44 @@ def test_fp32_form_vs_reference_float_build(model, oracle):
               f"mixed {mixed:.2e}")
         assert ours32 <= ref32
         assert mixed < 1e-5
+
+
+def _load_file(model, path):
+    model.cp3m_load_earth_file.argtypes = [C.c_char_p, dp, C.c_int,
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.c_char_p, C.c_int]
+    rows = np.zeros((64, 5))
+    n, poly = C.c_int(), C.c_int()
+    err = C.create_string_buffer(256)
+    rc = model.cp3m_load_earth_file(str(path).encode(), rows.ctypes.data_as(dp), 64,
+                                    C.byref(n), C.byref(poly), err, 256)
+    return rc, rows[:n.value], poly.value, err.value.decode()
+
+
+def test_density_file_formats(model, tmp_path):
+    """setDensityFromFile's parsing (atmoscpupropagator.hpp:250-344) in the
+    product's host code: comments and repeated spaces, the 3-column and the
+    5-column (polynomial) formats, rows stored descending; 4 columns are
+    refused with a message instead of ending the process."""
+    ep = W.prem4_poly()
+    p5 = tmp_path / "poly.dat"
+    with open(p5, "w") as f:
+        f.write("# radius a b c Ye\n")
+        for row in zip(ep.radii, ep.a, ep.b, ep.c, ep.yps):
+            f.write("   ".join(repr(float(x)) for x in row) + "\n")
+    rc, rows, poly, _ = _load_file(model, p5)
+    assert rc == 0 and poly == 1
+    expect = np.stack([ep.radii, ep.a, ep.b, ep.c, ep.yps], axis=1)[::-1]
+    assert np.array_equal(rows, expect)
+    e = W.prem4()
+    p3 = tmp_path / "const.dat"
+    with open(p3, "w") as f:
+        for r, d, y in zip(e.radii, e.rhos, e.yps):
+            f.write(f"{float(r)!r}  {float(d)!r} {float(y)!r}\n")
+    rc, rows, poly, _ = _load_file(model, p3)
+    assert rc == 0 and poly == 0
+    assert np.array_equal(rows[:, [0, 1, 4]], np.stack([e.radii, e.rhos, e.yps], 1)[::-1])
+    p4 = tmp_path / "bad.dat"
+    p4.write_text("0 1 2 3\n6371 1 2 3\n")
+    rc, _, _, msg = _load_file(model, p4)
+    assert rc == -1 and "4 entries per line" in msg
