@@ -89,6 +89,52 @@ def time_steps(b, steps, warmup, graph):
     return (t1 - t0) * 1e3 / steps, np.array(kernel_ms), mean_ph
 
 
+def other_configs():
+    """Kernel time of the other throughput configurations, median of 5 runs
+    after one warm-up: the single-grid propagator on the 12-shell model at
+    2000 x 2000 (0.1-100 GeV, nominal parameters, neutrinos; FP64, mixed, FP32)
+    and the beam batch of 4096 seeded sets x 1e5 energies (0.1-20 GeV, 1300 km,
+    2.6 g/cm^3) in one mixed launch."""
+    from cudaprob3_amd import AtmosPropagator
+    from cudaprob3_amd.batch import BEAM
+    out = {}
+    e = W.earth_12()
+    n = 2000
+    cz, ev = W.cos_grid(n), W.log_grid(n, 0.1, 100.0)
+    t12, t13, t23, d, k, m12, m23 = W.NOMINAL.for_nutype(0)
+    single = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        with AtmosPropagator(n, n, precision=prec) as p:
+            p.setEnergyList(ev)
+            p.setCosineList(cz)
+            p.setMNSMatrix(t12, t13, t23, d, k)
+            p.setNeutrinoMasses(m12, m23)
+            p.setDensity(e.radii, e.rhos, e.yps)
+            p.setProductionHeight(W.PROD_HEIGHT_KM)
+            ms = []
+            for _ in range(6):
+                p.calculateProbabilities(0)
+                ms.append(p.lastKernelMs())
+            t = float(np.median(ms[1:]))
+            single[prec] = {"kernel_ms": t, "evals_per_s": n * n / (t / 1e3)}
+    out["single_grid_12shell_2000x2000"] = single
+    bev = W.log_grid(100000, 0.1, 20.0)
+    sets = W.draw_param_sets(4096, 4096)
+    with Batch("mixed") as b:
+        g = b.add_beam_grid(bev, 2.6, 1300.0)
+        b.set_jobs([make_job(BEAM, g, s, 0) for s in sets])
+        b.use_graph(True)
+        ms = []
+        for _ in range(6):
+            b.run()
+            b.wait()
+            ms.append(b.last_ms())
+        t = float(np.median(ms[1:]))
+    out["beam_4096sets_1e5E_mixed"] = {"kernel_ms": t,
+                                       "evals_per_s": 4096 * 1e5 / (t / 1e3)}
+    return out
+
+
 def kernel_name(precision, tuning):
     """The batch's dominant kernel (the atmospheric propagation)."""
     pol = {"mixed": "PolicyMixed", "fp64": "PolicyF64", "fp32": "PolicyF32"}[precision]
@@ -171,6 +217,9 @@ def main():
     ap.add_argument("--sets", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the single-grid (12 shells, 2000 x 2000) and beam "
+                         "(4096 sets x 1e5 energies) timings")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -314,6 +363,9 @@ def main():
                              for p in ("mixed", "fp64")}
         result["p50_us"] = result["serving"]["mixed"]["p50_us"]
         result["p99_us"] = result["serving"]["mixed"]["p99_us"]
+
+    if not args.no_configs:
+        result["configs"] = other_configs()
 
     if not args.no_cpu:
         cb, worst, rms = cpu_baseline(wl, mixed_results)
