@@ -122,6 +122,7 @@ _SIGS.update({
     "cp3_server_create": (C.c_int, [C.c_int, C.c_int, C.c_size_t, C.c_int, C.c_int, C.POINTER(_vp)]),
     "cp3_server_destroy": (C.c_int, [_vp]),
     "cp3_server_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "cp3_server_set_solve_mode": (C.c_int, [_vp, C.c_int]),
     "cp3_server_add_atmos_grid": (C.c_int, [_vp, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, _ip]),
     "cp3_server_add_beam_grid": (C.c_int, [_vp, _dp, C.c_int, C.c_double, C.c_double, _ip]),
     "cp3_server_submit": (C.c_int, [_vp, C.POINTER(Cp3Job), _i64p]),

@@ -98,7 +98,9 @@ enum ReqState { kPending, kRunning, kDone, kFailed };
 
 struct Request {
   cp3_job job{};
-  size_t bytes = 0, off = 0;
+  // bytes: the result at off; tbytes: the slab table right after it (at
+  // off + round(bytes)) while the request's batch runs, 0 otherwise
+  size_t bytes = 0, off = 0, tbytes = 0;
   int64_t evals = 0;
   Clock::time_point t_submit, t_launch, t_done;
   ReqState state = kPending;
@@ -126,6 +128,9 @@ double us_between(Clock::time_point a, Clock::time_point b) {
 struct cp3_server {
   int dev = 0, precision = CP3_MIXED;
   int nc = 2, w = 4;
+  // atmospheric batches pre-solve their slabs into a table (CP3_SOLVE_TABLE,
+  // the default) or solve inside the propagation kernel (CP3_SOLVE_SHARED)
+  int solve_mode = CP3_SOLVE_TABLE;
   int max_batch = 256;
   DevBuf arena;
   std::unique_ptr<ArenaAlloc> alloc;
@@ -178,6 +183,13 @@ struct cp3_server {
     if (j.nutype != 0 && j.nutype != 1) throw Error(CP3_ERR_INVALID, "neutrino type must be 0 or 1");
   }
 
+  // Slab-table bytes of an atmospheric job's grid.
+  size_t table_bytes(const cp3_job& j) const {
+    if (j.kind != CP3_JOB_ATMOS) return 0;
+    return (size_t)cp3::slab_entries(agrids[(size_t)j.grid]->dev()) *
+           cp3::slab_entry_bytes(precision);
+  }
+
   size_t points(const cp3_job& j) const {
     return j.kind == CP3_JOB_ATMOS ? agrids[(size_t)j.grid]->points()
                                    : bgrids[(size_t)j.grid]->points();
@@ -187,11 +199,14 @@ struct cp3_server {
   // fit in the arena (it waits for memory; admission stays FIFO).
   bool launch_batch(Slot& s) {
     std::vector<int64_t> take;
+    const bool table = solve_mode == CP3_SOLVE_TABLE;
     while (!pending.empty() && (int)take.size() < max_batch) {
       Request& r = reqs[pending.front()];
+      const size_t tb = table ? table_bytes(r.job) : 0;
       size_t off = 0;
-      if (!alloc->alloc(r.bytes, &off)) break;
+      if (!alloc->alloc(ArenaAlloc::round(r.bytes) + ArenaAlloc::round(tb), &off)) break;
       r.off = off;
+      r.tbytes = tb;
       r.has_mem = true;
       take.push_back(pending.front());
       pending.pop_front();
@@ -200,7 +215,7 @@ struct cp3_server {
     peak_mem = std::max(peak_mem, alloc->used());
     s.ajobs.clear();
     s.bjobs.clear();
-    int64_t au = 0, bu = 0;
+    int64_t au = 0, bu = 0, max_waves = 0;
     for (int64_t id : take) {
       const Request& r = reqs[id];
       const cp3_job& in = r.job;
@@ -215,7 +230,13 @@ struct cp3_server {
         a.out = out;
         a.unit_begin = au;
         a.etiles = cp3::energy_tiles(a.g);
-        au += cp3::atmos_units(a.g, nc, w);
+        if (table) {
+          a.slabs = (char*)out + ArenaAlloc::round(r.bytes);
+          au += cp3::atmos_units(a.g, nc, 1);
+          max_waves = std::max(max_waves, cp3::slab_waves(a.g));
+        } else {
+          au += cp3::atmos_units(a.g, nc, w);
+        }
         s.ajobs.push_back(a);
       } else {
         const BeamGrid& g = *bgrids[(size_t)in.grid];
@@ -244,11 +265,21 @@ struct cp3_server {
         hip_check(hipMemcpyAsync(s.d_bjobs.p, (char*)s.staging.p + abytes, bbytes,
                                  hipMemcpyHostToDevice, s.stream),
                   "job table upload");
-      if (au > 0)
-        hip_check(cp3::launch_atmos_batch(precision, nc, w, (const cp3::AtmosJob*)s.d_ajobs.p,
-                                          (int)s.ajobs.size(), au,
+      const cp3::AtmosJob* aj = (const cp3::AtmosJob*)s.d_ajobs.p;
+      const int na = (int)s.ajobs.size();
+      if (au > 0 && table) {
+        hip_check(cp3::launch_slab_table(precision, aj, na, max_waves, s.stream),
+                  "slab table launch");
+        // one workgroup per unit, a multiple of 8 (one share per XCD)
+        hip_check(cp3::launch_atmos_table(precision, nc, aj, na, au,
+                                          (int)std::min<int64_t>((au + 7) / 8 * 8, 0x7ffffff8),
+                                          s.stream),
+                  "atmospheric launch");
+      } else if (au > 0) {
+        hip_check(cp3::launch_atmos_batch(precision, nc, w, aj, na, au,
                                           (int)std::min<int64_t>(au, 0x7fffffff), s.stream),
                   "atmospheric launch");
+      }
       if (bu > 0)
         hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)s.d_bjobs.p,
                                          (int)s.bjobs.size(), bu,
@@ -256,6 +287,8 @@ struct cp3_server {
                   "beam launch");
       hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
     } catch (const std::exception& e) {
+      // work enqueued before the failure may still use the requests' memory
+      (void)hipStreamSynchronize(s.stream);
       for (int64_t id : take) fail(id, e.what());
       cv_done.notify_all();
       return true;
@@ -280,8 +313,18 @@ struct cp3_server {
     r.state = kFailed;
     r.err = why;
     if (r.has_mem) {
-      alloc->release(r.off, r.bytes);
+      alloc->release(r.off, ArenaAlloc::round(r.bytes) + ArenaAlloc::round(r.tbytes));
+      r.tbytes = 0;
       r.has_mem = false;
+    }
+  }
+
+  // The slab table of a finished request goes back to the arena; its result
+  // stays until fetched or released.
+  void drop_table(Request& r) {
+    if (r.has_mem && r.tbytes) {
+      alloc->release(r.off + ArenaAlloc::round(r.bytes), r.tbytes);
+      r.tbytes = 0;
     }
   }
 
@@ -291,6 +334,7 @@ struct cp3_server {
       auto it = reqs.find(id);
       if (it == reqs.end()) continue;
       if (status == hipSuccess) {
+        drop_table(it->second);
         it->second.state = kDone;
         it->second.t_done = now;
         ++n_done;
@@ -428,6 +472,16 @@ int cp3_server_set_tuning(cp3_server* s, int nc, int w) {
   });
 }
 
+int cp3_server_set_solve_mode(cp3_server* s, int mode) {
+  return guard([&] {
+    need(s);
+    if (mode != CP3_SOLVE_AUTO && mode != CP3_SOLVE_SHARED && mode != CP3_SOLVE_TABLE)
+      throw Error(CP3_ERR_INVALID, "solve mode must be CP3_SOLVE_AUTO, _SHARED or _TABLE");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->solve_mode = mode == CP3_SOLVE_SHARED ? CP3_SOLVE_SHARED : CP3_SOLVE_TABLE;
+  });
+}
+
 int cp3_server_add_atmos_grid(cp3_server* s, const double* cosv, int nC, const double* ev,
                               int nE, const double* radii, const double* rhos,
                               const double* yps, int nrows, double h_km, int* grid_id) {
@@ -466,9 +520,11 @@ int cp3_server_submit(cp3_server* s, const cp3_job* job, int64_t* request_id) {
     r.job = *job;
     r.evals = (int64_t)s->points(*job);
     r.bytes = (size_t)r.evals * 9 * (size_t)s->elem();
-    if (ArenaAlloc::round(r.bytes) > s->alloc->cap())
-      throw Error(CP3_ERR_CAPACITY, "request output (" + std::to_string(r.bytes) +
-                                        " B) exceeds the server's memory cap");
+    // the result and, for an atmospheric request, its slab table while it runs
+    const size_t need = ArenaAlloc::round(r.bytes) + ArenaAlloc::round(s->table_bytes(*job));
+    if (need > s->alloc->cap())
+      throw Error(CP3_ERR_CAPACITY, "request output and table (" + std::to_string(need) +
+                                        " B) exceed the server's memory cap");
     r.t_submit = Clock::now();
     const int64_t id = s->next_id++;
     s->reqs.emplace(id, std::move(r));

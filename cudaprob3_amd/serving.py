@@ -58,6 +58,11 @@ class Server:
         except Exception:
             pass
 
+    def set_solve_mode(self, mode="table"):
+        """"table" (slabs pre-solved per request, the default) or "shared"
+        (solved inside the propagation kernel); results do not depend on it."""
+        N.check(self._lib.cp3_server_set_solve_mode(self._h, N.SOLVE_MODES[mode]))
+
     def set_tuning(self, cosine_block, group_waves):
         N.check(self._lib.cp3_server_set_tuning(self._h, int(cosine_block),
                                                 int(group_waves)))

@@ -289,6 +289,12 @@ int cp3_server_create(int device, int precision, size_t mem_cap_bytes,
 int cp3_server_destroy(cp3_server* s);
 /* Kernel tuning for every batch (cosine block, group waves). */
 int cp3_server_set_tuning(cp3_server* s, int cosine_block, int group_waves);
+/* Solve mode of the atmospheric batches: CP3_SOLVE_TABLE (and AUTO, the
+ * default) pre-solves each request's slabs into a table that lives in the
+ * memory cap while the request runs ((shells + 1) x energies x 40 B in mixed
+ * precision); CP3_SOLVE_SHARED solves inside the propagation kernel.  Results
+ * do not depend on it (bitwise).  No counterpart in the reference. */
+int cp3_server_set_solve_mode(cp3_server* s, int mode);
 int cp3_server_add_atmos_grid(cp3_server* s, const double* cosines,
                               int n_cosines, const double* energies,
                               int n_energies, const double* radii_km,

@@ -29,8 +29,10 @@ def single_atmos(n, params, nutype, precision):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_alone_equals_in_mix(precision):
+@pytest.mark.parametrize("mode", ["table", "shared"])
+def test_alone_equals_in_mix(precision, mode):
     with Server(precision, mem_cap_bytes=2 << 30, slots=3) as srv:
+        srv.set_solve_mode(mode)
         agrid, bgrid = serving_setup(srv)
         p = W.draw_param_sets(1, 4242)[0]
         probe_a = make_job(ATMOS, agrid[250], p, 1)
@@ -70,7 +72,7 @@ def test_memory_cap_respected():
         st = srv.stats()
         assert st["completed"] == 10
         assert 0 < st["mem_peak"] <= cap
-        with pytest.raises(Cp3Error, match="exceeds the server's memory cap"):
+        with pytest.raises(Cp3Error, match="the server's memory cap"):
             srv.submit(make_job(ATMOS, agrid[1000], sets[0], 0))
 
 
