@@ -527,12 +527,33 @@ void set_density_file(Earth& e, const std::string& path) {
   set_density(e, r, d, y);
 }
 
-// physics.hpp:789-1150 (calculate_atmos, no production height averaging;
-// constant or polynomial densities, :915-920).  out is [cos][E][9], channel
-// a*3+b = P(a->b).
+// Production-height averaging input (physics.hpp:803-807): nb bins, bin
+// probabilities prob[type][flavour][E][cos][bin] (the layout indexed at
+// physics.hpp:1042-1043) and nb + 1 bin edges in km (multiplied by km2cm at
+// physics.hpp:1013).  `int_abs_sinc` reproduces the compiled reference's
+// defined_sinc (math.hpp:49-55): there g++ resolves the unqualified abs() to
+// int abs(int), so the series branch is taken for every |A| < 1 instead of
+// |A| < 1e-6.  Without it the sinc is the intended one.
+struct Pha {
+  int nb;
+  const double* prob;
+  const double* edges_km;
+  bool int_abs_sinc;
+};
+
+// math.hpp:49-55 (defined_sinc), Epsilon() = 1e-6 (constants.hpp:49).
+static inline double sinc(double A, bool int_abs) {
+  const double mag = int_abs ? (double)std::abs((int)A) : std::fabs(A);
+  if (mag >= 1e-6) return std::sin(A) / A;
+  return 1.0 - A * A / 6. + A * A * A * A / 120.;
+}
+
+// physics.hpp:789-1150 (calculate_atmos; constant or polynomial densities,
+// :915-920; production-height averaging when pha is given, :997-1067).  out is
+// [cos][E][9], channel a*3+b = P(a->b).
 void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
                      const double* cosv, int nC, const double* ev, int nE,
-                     double h_cm, double* out) {
+                     double h_cm, double* out, const Pha* pha = nullptr) {
   const int nrows = (int)earth.radii.size();
   std::vector<int> maxl((size_t)nC);
   for (int c = 0; c < nC; ++c) maxl[(size_t)c] = max_layer(earth, cosv[c]);
@@ -546,6 +567,7 @@ void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
     for (int ie = 0; ie < nE; ++ie) {
       const double E = ev[ie];
       cx fin[3][3], ctm[3][3], T[3][3], tmp[3][3];
+      double darg0[3] = {0., 0., 0.};
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) ctm[i][j] = {i == j ? 1.0 : 0.0, 0.0};
       const double path_cm =
@@ -566,6 +588,8 @@ void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
         if (layer == 0) {
           std::memcpy(fin, T, sizeof(fin));
           std::memcpy(C_atm, C, sizeof(C_atm));
+          // physics.hpp:958-967: phase per cm of the atmosphere segment
+          for (int k = 0; k < 3; ++k) darg0[k] = dist == 0. ? 0. : arg[k] / dist;
         } else if (layer < M) {
           mul(T, fin, tmp);
           std::memcpy(fin, tmp, sizeof(fin));
@@ -579,8 +603,52 @@ void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
       mul(ctm, fin, tmp);
       std::memcpy(fin, tmp, sizeof(fin));
 
+      // physics.hpp:997-1067: length-shift factors S[k][j][flavour before],
+      // 1 (off the diagonal) without averaging.
+      cx S[3][3][3];
+      if (pha) {
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j)
+            for (int f = 0; f < 3; ++f) S[i][j][f] = {i == j ? 1.0 : 0.0, 0.};
+        std::vector<double> shift((size_t)pha->nb + 1);
+        for (int b = 0; b < pha->nb + 1; ++b) {
+          const double hb = kKm2Cm * pha->edges_km[b];
+          const double pl = (std::sqrt((kREarthCm + hb) * (kREarthCm + hb) -
+                                       (kREarthCm * kREarthCm) * (1 - cz * cz)) -
+                             kREarthCm * cz);
+          shift[(size_t)b] = pl - path_cm;
+        }
+        const size_t nb = (size_t)pha->nb;
+        for (int bin = 0; bin < pha->nb; ++bin) {
+          const double h0 = shift[(size_t)bin], h1 = shift[(size_t)bin + 1];
+          const double hm = (h1 + h0) / 2.;
+          const double hw = (h1 - h0);
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < i; ++j) {
+              const double dd = darg0[i] - darg0[j];
+              const double A = 0.5 * dd * hw;
+              const cx f = {sinc(A, pha->int_abs_sinc) * std::cos(dd * hm),
+                            sinc(A, pha->int_abs_sinc) * std::sin(dd * hm)};
+              for (int fl = 0; fl < 3; ++fl) {
+                const size_t idx = (size_t)(anti ? 1 : 0) * 3 * (size_t)nE * nC * nb +
+                                   (size_t)fl * (size_t)nE * nC * nb +
+                                   (size_t)ie * nC * nb + (size_t)ic * nb + (size_t)bin;
+                const double w = pha->prob[idx];
+                S[i][j][fl].re += w * f.re;
+                S[i][j][fl].im += w * f.im;
+                S[j][i][fl].re += w * f.re;
+                S[j][i][fl].im -= w * f.im;
+              }
+            }
+        }
+      } else {
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < i; ++j)
+            for (int f = 0; f < 3; ++f) S[i][j][f] = {1.0, 0.};
+      }
+
       // physics.hpp:1102-1130: P(a->b) assembled from the expansion of the
-      // atmosphere layer; the length-shift factor is 1 without averaging.
+      // atmosphere layer.
       cx prod[3][3][3];
       for (int k = 0; k < 3; ++k) mul(fin, C_atm[k], prod[k]);
       double prob[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
@@ -592,10 +660,10 @@ void calculate_atmos(const Osc& o, const Earth& earth, bool anti,
         for (int j = 0; j < k; ++j)
           for (int b = 0; b < 3; ++b)
             for (int a = 0; a < 3; ++a)
-              prob[a][b] += 2. * prod[j][a][b].re * prod[k][a][b].re * 1.0 +
-                            2. * prod[j][a][b].im * prod[k][a][b].im * 1.0 +
-                            2. * prod[j][a][b].im * prod[k][a][b].re * 0.0 -
-                            2. * prod[j][a][b].re * prod[k][a][b].im * 0.0;
+              prob[a][b] += 2. * prod[j][a][b].re * prod[k][a][b].re * S[k][j][b].re +
+                            2. * prod[j][a][b].im * prod[k][a][b].im * S[k][j][b].re +
+                            2. * prod[j][a][b].im * prod[k][a][b].re * S[k][j][b].im -
+                            2. * prod[j][a][b].re * prod[k][a][b].im * S[k][j][b].im;
       }
       double* dst = out + ((size_t)ic * nE + ie) * 9;
       for (int a = 0; a < 3; ++a)
@@ -698,6 +766,36 @@ int orc_atmos_f64(int nutype, const double* cosv, int nC, const double* ev,
     orc::set_threads(threads);
     orc::calculate_atmos(o, e, nutype != 0, cosv, nC, ev, nE,
                          h_km * 100000.0, out);
+  } catch (const std::exception& ex) {
+    orc::put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// The same with production-height averaging (physics.hpp:997-1067): nb bins,
+// prob[2][3][nE][nC][nb] (neutrino type, initial flavour, energy, cosine, bin)
+// and edges_km[nb + 1]; int_abs_sinc selects the compiled reference's sinc.
+int orc_atmos_pha_f64(int nutype, const double* cosv, int nC, const double* ev,
+                      int nE, const double* radii, const double* rhos,
+                      const double* yps, int nrows, double th12, double th13,
+                      double th23, double dcp, int kNuType, double dm12,
+                      double dm23, double h_km, int nb, const double* prob,
+                      const double* edges_km, int int_abs_sinc, int threads,
+                      double* out, char* err, int errlen) {
+  try {
+    if (nb < 0 || nb > 20)
+      throw std::runtime_error("invalid number of production height bins");
+    orc::Osc o;
+    orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+    orc::Earth e;
+    orc::set_density(e, std::vector<double>(radii, radii + nrows),
+                     std::vector<double>(rhos, rhos + nrows),
+                     std::vector<double>(yps, yps + nrows));
+    orc::set_threads(threads);
+    const orc::Pha pha{nb, prob, edges_km, int_abs_sinc != 0};
+    orc::calculate_atmos(o, e, nutype != 0, cosv, nC, ev, nE,
+                         h_km * 100000.0, out, &pha);
   } catch (const std::exception& ex) {
     orc::put_err(err, errlen, ex.what());
     return -1;

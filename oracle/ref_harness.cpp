@@ -115,6 +115,51 @@ int run_atmos(int nutype, const double* cosv, int nC, const double* ev, int nE,
   return 0;
 }
 
+// The reference class never turns production-height averaging on: the flag
+// defaults to false (atmoscpupropagator.hpp:532) and
+// SetNumberOfProductionHeightBinsForAveraging clears it for every n >= 1
+// (:97-99).  This subclass sets the protected flag after that call, so that
+// the reference's own setProductionHeightList (:395-427), calculateProbabilities
+// (:445-477) and averaging arithmetic (physics.hpp:997-1067, 1106-1130) run.
+template <typename T>
+struct AveragingPropagator : cudaprob3linear::AtmosCpuPropagator<T> {
+  AveragingPropagator(int nC, int nE, int threads)
+      : cudaprob3linear::AtmosCpuPropagator<T>(nC, nE, threads) {}
+  void enableAveraging() { this->useProductionHeightAveraging = true; }
+};
+
+int run_atmos_pha(int nutype, const double* cosv, int nC, const double* ev, int nE,
+                  const double* radii, const double* rhos, const double* yps,
+                  int nrows, double th12, double th13, double th23, double dcp,
+                  int kNuType, double dm12, double dm23, double h_km, int nb,
+                  const double* prob, const double* edges_km, int threads,
+                  double* out, char* err, int errlen) {
+  using namespace cudaprob3linear;
+  try {
+    AveragingPropagator<double> p(nC, nE, threads);
+    p.setEnergyList(to_vec<double>(ev, nE));
+    p.setCosineList(to_vec<double>(cosv, nC));
+    p.setMNSMatrix(th12, th13, th23, dcp, kNuType);
+    p.setNeutrinoMasses(dm12, dm23);
+    p.setDensity(to_vec<double>(radii, nrows), to_vec<double>(rhos, nrows),
+                 to_vec<double>(yps, nrows));
+    p.setProductionHeight(h_km);
+    p.SetNumberOfProductionHeightBinsForAveraging(nb);
+    p.enableAveraging();
+    p.setProductionHeightList(to_vec<double>(prob, nb * 2 * 3 * nE * nC),
+                              to_vec<double>(edges_km, nb + 1));
+    p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
+    for (int c = 0; c < nC; ++c)
+      for (int e = 0; e < nE; ++e)
+        for (int t = 0; t < 9; ++t)
+          out[((size_t)c * nE + e) * 9 + t] = p.getProbability(c, e, (ProbType)t);
+  } catch (const std::exception& ex) {
+    put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
 // Runs the reference beam CPU propagator in the call order of
 // example/mainbeam.cpp:113-133.
 template <typename T, typename OUT>
@@ -192,6 +237,21 @@ int ref_atmos_poly_f64(int nutype, const double* cosv, int nC, const double* ev,
   return run_atmos<double>(nutype, cosv, nC, ev, nE, radii, nullptr, yps, nrows,
                            nullptr, poly, th12, th13, th23, dcp, kNuType, dm12,
                            dm23, h_km, threads, out, err, errlen);
+}
+
+// Atmospheric probabilities with production-height averaging on: nb bins,
+// prob[2][3][nE][nC][nb] and edges_km[nb + 1] as setProductionHeightList takes
+// them.
+int ref_atmos_pha_f64(int nutype, const double* cosv, int nC, const double* ev,
+                      int nE, const double* radii, const double* rhos,
+                      const double* yps, int nrows, double th12, double th13,
+                      double th23, double dcp, int kNuType, double dm12,
+                      double dm23, double h_km, int nb, const double* prob,
+                      const double* edges_km, int threads, double* out, char* err,
+                      int errlen) {
+  return run_atmos_pha(nutype, cosv, nC, ev, nE, radii, rhos, yps, nrows, th12, th13,
+                       th23, dcp, kNuType, dm12, dm23, h_km, nb, prob, edges_km,
+                       threads, out, err, errlen);
 }
 
 int ref_beam_f64(int nutype, const double* ev, int nE, double rho, double L_km,

@@ -2,7 +2,7 @@
 propagators (oracle/_ref/libcp3ref*.so, compiled from the reference headers by
 oracle/Makefile).  Run in a container that has the reference sources:
 
-    make -C oracle && python tests/golden/make_golden.py [--only-poly]
+    make -C oracle && python tests/golden/make_golden.py [--only-poly | --only-pha]
 
 Each fixture is an .npz of plain float64/int arrays (inputs and expected
 outputs, [cos][E][9] or [E][9], channel a*3+b = P(a->b)); manifest.json
@@ -56,15 +56,44 @@ def poly_golden(ref, manifest):
             "nutype": nt, "lib": "libcp3ref.so", "bytes": size}
 
 
-def only_poly():
-    """Adds G6 to an existing set without rewriting the other fixtures."""
+def pha_golden(ref, manifest):
+    """G7: production-height averaging (physics.hpp:997-1067), which the
+    reference class never enables; the harness's subclass sets its protected
+    flag (oracle/ref_harness.cpp).  4-shell model, 16 x 16 points (cos from -1
+    to 0.875, downgoing included), h 22 km, 5 bins with edges 5-50 km and
+    seeded bin probabilities normalised per (type, flavour, E, cos)."""
+    e = W.prem4()
+    cz = -1.0 + np.arange(16, dtype=np.float64) / 8
+    ev = W.log_grid(16, 0.1, 100.0)
+    edges = np.array([5.0, 10.0, 15.0, 22.0, 30.0, 50.0])
+    rng = np.random.default_rng(997)
+    prob = rng.random((2, 3, len(ev), len(cz), len(edges) - 1))
+    prob /= prob.sum(axis=-1, keepdims=True)
+    outs = [ref.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps,
+                          osc_of(W.NOMINAL, nt), W.PROD_HEIGHT_KM, prob, edges)
+            for nt in (0, 1)]
+    name = "g7_prem4_height_avg"
+    size = save(name, cosv=cz, ev=ev, radii=e.radii, rhos=e.rhos, yps=e.yps,
+                edges_km=edges, height_prob=prob, prob=np.stack(outs))
+    manifest["fixtures"][name] = {
+        "workload": "4 shells, 16x16, cos -1+i/8, E 0.1-100 GeV log, h 22 km, "
+                    "production-height averaging over 5 bins (edges 5-50 km), "
+                    "bin probabilities seed 997",
+        "prob_axes": "nutype, cos, E, channel",
+        "height_prob_axes": "nutype, flavour, E, cos, bin",
+        "lib": "libcp3ref.so", "bytes": size}
+
+
+def only(fn):
+    """Adds one fixture group to an existing set without rewriting the
+    other fixtures."""
     ref = load_ref(deep=False)
     if ref is None:
         sys.exit("oracle/_ref is not built (needs the reference sources)")
     path = os.path.join(HERE, "manifest.json")
     with open(path) as f:
         manifest = json.load(f)
-    poly_golden(ref, manifest)
+    fn(ref, manifest)
     with open(path, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 
@@ -209,6 +238,7 @@ def main():
             "lib": "libcp3ref.so", "bytes": size}
 
     poly_golden(ref, manifest)
+    pha_golden(ref, manifest)
 
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
@@ -218,6 +248,8 @@ def main():
 
 if __name__ == "__main__":
     if sys.argv[1:] == ["--only-poly"]:
-        only_poly()
+        only(poly_golden)
+    elif sys.argv[1:] == ["--only-pha"]:
+        only(pha_golden)
     else:
         main()

@@ -347,3 +347,162 @@ def test_poly_constant_profile_equals_constant_model(oracle):
                               osc_for(NOM, nt), 22.0)
         y = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(NOM, nt), 22.0)
         assert np.abs(x - y).max() < 1e-13
+
+
+# ------------------------------------------- production-height averaging
+
+def _bin_prob(nC, nE, nb, seed):
+    """Seeded bin probabilities [2][3][nE][nC][nb], normalised per
+    (type, flavour, E, cos)."""
+    p = np.random.default_rng(seed).random((2, 3, nE, nC, nb))
+    return p / p.sum(axis=-1, keepdims=True)
+
+
+def test_height_avg_golden(oracle):
+    """The reference's averaging arithmetic (tests/golden/g7_*, made through
+    the compiled reference with its averaging flag set) reproduced with the
+    compiled reference's sinc, whose abs() truncates to an integer."""
+    g = golden("g7_prem4_height_avg")
+    for nt in (0, 1):
+        out = oracle.atmos_pha(nt, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                               g["yps"], osc_for(NOM, nt), 22.0,
+                               g["height_prob"], g["edges_km"],
+                               int_abs_sinc=True)
+        assert np.abs(out - g["prob"][nt]).max() <= TOL_GOLD
+        # the intended sinc moves results by at most a few 1e-5 here
+        fixed = oracle.atmos_pha(nt, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                                 g["yps"], osc_for(NOM, nt), 22.0,
+                                 g["height_prob"], g["edges_km"])
+        assert 0 < np.abs(fixed - g["prob"][nt]).max() < 1e-4
+
+
+def test_height_avg_reference_drawn_sets(oracle):
+    """Bitwise against the compiled reference for drawn parameter sets and
+    bin layouts, where it is built."""
+    from oracle.loader import load_ref
+    ref = load_ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    e = W.prem4()
+    cz = np.linspace(-1.0, 1.0, 11)
+    ev = W.log_grid(9, 0.1, 50.0)
+    for k, p in enumerate(W.draw_param_sets(3, 997)):
+        row = [p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+        nb = 1 + 3 * k
+        edges = np.sort(np.random.default_rng(k).uniform(1.0, 60.0, nb + 1))
+        prob = _bin_prob(len(cz), len(ev), nb, 50 + k)
+        for nt in (0, 1):
+            a = ref.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(row, nt),
+                              15.0, prob, edges)
+            b = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps,
+                                 osc_for(row, nt), 15.0, prob, edges,
+                                 int_abs_sinc=True)
+            assert np.array_equal(a, b)
+
+
+def test_height_avg_zero_width_bin_is_that_height(oracle):
+    """One bin of zero width at height h' with probability 1 moves the
+    production point: the result is the plain propagation from h'."""
+    e = W.prem4()
+    cz = np.linspace(-1.0, 1.0, 13)
+    ev = W.log_grid(11, 0.1, 30.0)
+    for h2 in (3.0, 22.0, 61.0):
+        prob = np.ones((2, 3, len(ev), len(cz), 1))
+        for nt in (0, 1):
+            avg = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps,
+                                   osc_for(NOM, nt), 22.0, prob, [h2, h2])
+            plain = oracle.atmos(nt, cz, ev, e.radii, e.rhos, e.yps,
+                                 osc_for(NOM, nt), h2)
+            assert np.abs(avg - plain).max() < 1e-12
+
+
+def _height_at_shift(cz, h_km, s_km):
+    """Production height whose path length exceeds that of h_km by s_km
+    (inverse of the path length of physics.hpp:897-898)."""
+    R = 6371.0
+    P0 = np.sqrt((R + h_km) ** 2 - R * R * (1 - cz * cz)) - R * cz
+    P = P0 + s_km
+    return np.sqrt((P + R * cz) ** 2 + R * R * (1 - cz * cz)) - R
+
+
+def test_height_avg_bin_is_the_mean_over_path_shifts(oracle):
+    """A bin averages the propagation uniformly over the path-length shifts
+    between its edges: one bin with probability 1 equals the 96-point
+    Gauss-Legendre mean of plain propagations from the heights of those
+    shifts.  The grid has sinc arguments between 1e-6 and 1, where the
+    compiled reference's truncating abs() takes the series instead."""
+    e = W.prem4()
+    cz = np.array([-0.9, -0.5, -0.1, 0.0, 0.2, 0.7])
+    ev = np.array([0.15, 0.6, 2.0, 8.0, 40.0])
+    h, edges = 22.0, np.array([12.0, 35.0])
+    x, wq = np.polynomial.legendre.leggauss(96)
+    prob = np.ones((2, 3, len(ev), len(cz), 1))
+    for nt in (0, 1):
+        osc = osc_for(NOM, nt)
+        avg = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps, osc, h,
+                               prob, edges)
+        quirk = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps, osc, h,
+                                 prob, edges, int_abs_sinc=True)
+        R = 6371.0
+        for i, c in enumerate(cz):
+            path = lambda hk: np.sqrt((R + hk) ** 2 - R * R * (1 - c * c)) - R * c
+            s0, s1 = path(edges[0]) - path(h), path(edges[1]) - path(h)
+            mean = np.zeros((len(ev), 9))
+            for xi, wi in zip(x, wq):
+                s = 0.5 * (s0 + s1) + 0.5 * (s1 - s0) * xi
+                hk = _height_at_shift(c, h, s)
+                mean += 0.5 * wi * oracle.atmos(nt, [c], ev, e.radii, e.rhos,
+                                                e.yps, osc, hk)[0]
+            assert np.abs(avg[i] - mean).max() < 1e-9
+        assert np.abs(quirk - avg).max() > 1e-6
+
+
+def test_height_avg_no_bins_is_incoherent(oracle):
+    """With averaging on and no bins the interference terms vanish: above
+    the horizon (vacuum only) P(a->b) = sum_k |U_ak|^2 |U_bk|^2; everywhere
+    each initial flavour's probabilities still sum to 1."""
+    e = W.prem4()
+    cz = np.linspace(-1.0, 1.0, 9)
+    ev = W.log_grid(7, 0.1, 30.0)
+    prob = np.zeros((2, 3, len(ev), len(cz), 0))
+    for nt in (0, 1):
+        osc = osc_for(NOM, nt)
+        out = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps, osc, 22.0,
+                               prob, [22.0])
+        U = pdg_u(osc.th12, osc.th13, osc.th23, osc.dcp * osc.knutype)
+        inc = (np.abs(U) ** 2) @ (np.abs(U) ** 2).T  # [a][b]
+        for i, c in enumerate(cz):
+            if c >= 0:
+                assert np.abs(out[i] - inc.ravel()).max() < 1e-12
+        assert np.abs(out.reshape(len(cz), len(ev), 3, 3).sum(-1) - 1).max() < 1e-12
+
+
+def test_height_avg_weights(oracle):
+    """Results are affine in the bin probabilities (normalised mixtures mix
+    the results), the probabilities of one initial flavour or neutrino type
+    touch only that flavour's channels or that type's run, and each initial
+    flavour's probabilities sum to 1."""
+    e = W.prem4()
+    cz = np.linspace(-1.0, 0.9, 8)
+    ev = W.log_grid(6, 0.1, 20.0)
+    edges = np.array([4.0, 9.0, 17.0, 26.0, 44.0])
+    p1 = _bin_prob(len(cz), len(ev), 4, 1)
+    p2 = _bin_prob(len(cz), len(ev), 4, 2)
+    osc = osc_for(NOM, 0)
+    run = lambda p, nt=0: oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps,
+                                           osc_for(NOM, nt), 22.0, p, edges)
+    a, b, m = run(p1), run(p2), run(0.3 * p1 + 0.7 * p2)
+    assert np.abs(m - (0.3 * a + 0.7 * b)).max() < 1e-13
+    assert np.abs(a.reshape(len(cz), len(ev), 3, 3).sum(-1) - 1).max() < 1e-12
+    q = p1.copy()
+    q[0, 1] = p2[0, 1]  # neutrino, initial mu
+    c = run(q)
+    mu = [3, 4, 5]
+    others = [0, 1, 2, 6, 7, 8]
+    assert np.array_equal(c[..., others], a[..., others])
+    assert np.abs(c[..., mu] - a[..., mu]).max() > 1e-6
+    q = p1.copy()
+    q[1] = p2[1]  # the antineutrino slice
+    assert np.array_equal(run(q), a)
+    assert not np.array_equal(run(q, 1), run(p1, 1))
+    del osc
