@@ -68,6 +68,9 @@ _SIGS = {
     "cp3_atmos_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
     "cp3_atmos_set_solve_mode": (C.c_int, [_vp, C.c_int]),
     "cp3_atmos_set_density_poly": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, C.c_int]),
+    "cp3_atmos_set_production_height_bins": (C.c_int, [_vp, C.c_int]),
+    "cp3_atmos_set_production_height_averaging": (C.c_int, [_vp, C.c_int]),
+    "cp3_atmos_set_production_height_list": (C.c_int, [_vp, _dp, C.c_int64, _dp, C.c_int]),
     "cp3_beam_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
     "cp3_beam_destroy": (C.c_int, [_vp]),
     "cp3_beam_set_energy_list": (C.c_int, [_vp, _dp, C.c_int]),

@@ -105,6 +105,14 @@ struct cp3_atmos {
   PinnedBuf h_out;
   int S = 0, rye_cs = 0;
   bool table_dirty = true, energies_dirty = true, results_on_host = false;
+  // production-height averaging (atmoscpupropagator.hpp:88-108, 395-427,
+  // 532-535): the flag, the bin count, the bin edges (km) and, on the device,
+  // the reordered bin probabilities of both neutrino types and the per-cosine
+  // bin shifts (rebuilt with the trajectory table)
+  bool avg = false, avg_list = false;
+  int n_bins = 0;
+  std::vector<double> edges_km;
+  DevBuf d_pha_w, d_pha_hmw;
 
   ~cp3_atmos() {
     DeviceScope ds(dev);
@@ -132,8 +140,15 @@ struct cp3_atmos {
       d_Lvac.upload(tt.Lvac.data(), tt.Lvac.size() * sizeof(double), dev);
       d_seg.upload(tt.seg.data(), tt.seg.size() * sizeof(double), dev);
       d_rye.upload(tt.rye.data(), tt.rye.size() * sizeof(double), dev);
+      if (avg_list) upload_bins();
       table_dirty = false;
     }
+  }
+
+  void upload_bins() {
+    std::vector<double> hmw;
+    cp3::build_height_bins(hmw, cosines.data(), nC, h_cm, edges_km);
+    if (!hmw.empty()) d_pha_hmw.upload(hmw.data(), hmw.size() * sizeof(double), dev);
   }
 
   cp3::GridDev grid() const {
@@ -155,6 +170,10 @@ struct cp3_atmos {
     if (!height_set)
       throw Error(CP3_ERR_STATE,
                   "CudaPropagatorSingle::calculateProbabilities. production height was not set");
+    if (avg && !avg_list)
+      throw Error(CP3_ERR_STATE,
+                  "CpuPropagator::calculateProbabilities. production height array was not set, "
+                  "but been requested to use production height averaging");
     DeviceScope ds(dev);
     upload_tables();
     cp3::AtmosJob job{};
@@ -167,7 +186,32 @@ struct cp3_atmos {
     // a polynomial model's densities differ per cosine: the shared solve with
     // one cosine per wave and one wave per workgroup solves per trajectory
     const bool per_cosine = job.g.rye_cs != 0;
-    if (solve_mode == CP3_SOLVE_SHARED || per_cosine) {
+    if (avg) {
+      // the bin probabilities of this neutrino type, one cosine per wave
+      const int64_t plane = (int64_t)nC * nE;
+      job.pha.w = (const char*)d_pha_w.p +
+                  (size_t)nutype * 3 * n_bins * plane * cp3::out_elem_bytes(precision);
+      job.pha.hmw = (const double*)d_pha_hmw.p;
+      job.pha.nb = n_bins;
+      if (solve_mode == CP3_SOLVE_SHARED || per_cosine) {
+        hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+        hip_check(cp3::launch_atmos_single_avg(precision, job, stream),
+                  "atmospheric kernel launch");
+      } else {
+        d_slabs.alloc((size_t)cp3::slab_entries(job.g) * cp3::slab_entry_bytes(precision),
+                      dev);
+        job.slabs = d_slabs.p;
+        d_job.alloc(sizeof(job), dev);
+        cp3::AtmosJob* dj = (cp3::AtmosJob*)d_job.p;
+        const int64_t units = cp3::cosine_blocks(job.g, 1, 1) * job.etiles;
+        hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+        hip_check(cp3::launch_slab_table_single(precision, job, dj, stream),
+                  "slab table launch");
+        hip_check(cp3::launch_atmos_table_avg(precision, dj, units,
+                                              (int)((units + 7) / 8 * 8), stream),
+                  "atmospheric kernel launch");
+      }
+    } else if (solve_mode == CP3_SOLVE_SHARED || per_cosine) {
       int nc = 1, w = 1;
       auto_tuning(precision, job.g, &nc, &w);
       if (nc_force) nc = nc_force;
@@ -463,6 +507,67 @@ int cp3_atmos_set_production_height(cp3_atmos* p, double km) {
       throw Error(CP3_ERR_STATE, "must set cosine list before production height");
     p->h_cm = km * 100000.0;
     p->height_set = true;
+    p->table_dirty = true;
+  });
+}
+
+int cp3_atmos_set_production_height_bins(cp3_atmos* p, int n_bins) {
+  return guard([&] {
+    need(p);
+    if (n_bins > 20)
+      throw Error(CP3_ERR_INVALID,
+                  "SetNumberOfProductionHeightBinsForAveraging : invalid number of production "
+                  "height bins");
+    if (n_bins < 0)
+      throw Error(CP3_ERR_INVALID, "negative number of production height bins");
+    p->n_bins = n_bins;
+    if (n_bins >= 1) p->avg = false;
+  });
+}
+
+int cp3_atmos_set_production_height_averaging(cp3_atmos* p, int on) {
+  return guard([&] {
+    need(p);
+    p->avg = on != 0;
+    p->avg_list = false;
+  });
+}
+
+int cp3_atmos_set_production_height_list(cp3_atmos* p, const double* prob, int64_t n_prob,
+                                         const double* edges_km, int n_edges) {
+  return guard([&] {
+    need(p);
+    if (!p->avg)
+      throw Error(CP3_ERR_STATE,
+                  "Propagator::setProductionHeightList. Trying to set Production Height "
+                  "information but propagator is not expecting to use it");
+    const int64_t want = (int64_t)p->n_bins * 2 * 3 * p->nE * p->nC;
+    if (n_prob != want)
+      throw Error(CP3_ERR_INVALID,
+                  "Propagator::setProductionHeightList. Prob array is not the expected size");
+    if (n_edges - 1 != p->n_bins)
+      throw Error(CP3_ERR_INVALID,
+                  "Propagator::setProductionHeightList. ProductionHeightBins array is not "
+                  "expected size");
+    if (n_prob > 0) need_ptr(prob, "prob");
+    need_ptr(edges_km, "edges_km");
+    DeviceScope ds(p->dev);
+    // the previous calculation may still read the old lists
+    hip_check(hipStreamSynchronize(p->stream), "hipStreamSynchronize");
+    if (n_prob > 0) {
+      if (p->precision == CP3_FP64) {
+        std::vector<double> w;
+        cp3::reorder_height_prob(w, prob, p->nC, p->nE, p->n_bins);
+        p->d_pha_w.upload(w.data(), w.size() * sizeof(double), p->dev);
+      } else {
+        std::vector<float> w;
+        cp3::reorder_height_prob(w, prob, p->nC, p->nE, p->n_bins);
+        p->d_pha_w.upload(w.data(), w.size() * sizeof(float), p->dev);
+      }
+    }
+    p->edges_km.assign(edges_km, edges_km + n_edges);
+    p->avg_list = true;
+    // the bin shifts follow the cosines and the production height
     p->table_dirty = true;
   });
 }

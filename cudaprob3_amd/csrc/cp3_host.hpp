@@ -408,7 +408,57 @@ inline double poly_layer_rye(const EarthModel& e, double cz, int layer, int max_
   const double i1 = (r_int_a(t2) - r_int_b(t2)) - (r_int_a(t1) - r_int_b(t1));
   const double i0 = t2 - t1;
   const double rho = (pa * i2 + pb * i1 + pc * i0) / (t2 - t1);
+  // A grazing trajectory (cz a little below 0, about -1e-9 and up) crosses the
+  // outer shell along a segment so short that these closed forms reach 0/0 or
+  // log(0), and the reference's density is NaN.  The mean over a vanishing
+  // segment is the density at the shell's outer radius.
+  if (!std::isfinite(rho)) return (pa * r2max + pb * Ri + pc) * e.yps[k];
   return rho * e.yps[k];
+}
+
+// Production-height bins of a cosine list (physics.hpp:1009-1033): the path
+// length from each of the nb + 1 edges (km) minus the one from h_cm, in cm
+// and in the reference's operation order, then per bin the mid shift and the
+// width, hmw[(c * nb + b) * 2 + {0, 1}], in km.
+inline void build_height_bins(std::vector<double>& hmw, const double* cosv, int nC,
+                              double h_cm, const std::vector<double>& edges_km) {
+  const int nb = (int)edges_km.size() - 1;
+  hmw.assign((size_t)nC * (size_t)(nb > 0 ? nb : 0) * 2, 0.0);
+  std::vector<double> shift(edges_km.size());
+  for (int c = 0; c < nC; ++c) {
+    const double cz = cosv[c];
+    const double path_cm = std::sqrt((kREarthCm + h_cm) * (kREarthCm + h_cm) -
+                                     (kREarthCm * kREarthCm) * (1 - cz * cz)) -
+                           kREarthCm * cz;
+    for (size_t b = 0; b < edges_km.size(); ++b) {
+      const double hb = kKm2Cm * edges_km[b];
+      const double pl = (std::sqrt((kREarthCm + hb) * (kREarthCm + hb) -
+                                   (kREarthCm * kREarthCm) * (1 - cz * cz)) -
+                         kREarthCm * cz);
+      shift[b] = pl - path_cm;
+    }
+    for (int b = 0; b < nb; ++b) {
+      const double h0 = shift[(size_t)b], h1 = shift[(size_t)b + 1];
+      double* d = &hmw[((size_t)c * nb + (size_t)b) * 2];
+      d[0] = (h1 + h0) / 2. / kKm2Cm;
+      d[1] = (h1 - h0) / kKm2Cm;
+    }
+  }
+}
+
+// The bin probabilities of setProductionHeightList, [type][flavour][E][cos][bin]
+// (physics.hpp:1042-1043), reordered to [type][flavour][bin][cos][E] so that
+// the lanes of a wave (consecutive energies) read consecutive entries.
+template <class T>
+void reorder_height_prob(std::vector<T>& out, const double* prob, int nC, int nE, int nb) {
+  const size_t plane = (size_t)nC * (size_t)nE;
+  out.resize((size_t)6 * (size_t)nb * plane);
+  for (int tf = 0; tf < 6; ++tf)
+    for (int e = 0; e < nE; ++e)
+      for (int c = 0; c < nC; ++c)
+        for (int b = 0; b < nb; ++b)
+          out[((size_t)tf * nb + (size_t)b) * plane + (size_t)c * nE + (size_t)e] =
+              (T)prob[(((size_t)tf * nE + (size_t)e) * nC + (size_t)c) * nb + (size_t)b];
 }
 
 inline void build_traj_table(TrajTable& t, const EarthModel& e,

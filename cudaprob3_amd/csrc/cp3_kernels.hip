@@ -148,13 +148,36 @@ __device__ __forceinline__ void cos_set(const GridDev& g, int base, CosSet<NC>& 
   }
 }
 
+// Production-height bins of one lane's trajectory (PhaDev): the cosine's
+// shifts through scalar loads, the lane's bin probabilities as coalesced rows.
+template <class P>
+struct DevBins {
+  using D = typename P::D;
+  using O = typename OutT<P>::type;
+  int n;
+  const double* hmw;  // this cosine's [nb][2]
+  const O* w;         // this (cosine, energy) entry of [3][nb][nC][nE]
+  int64_t plane;
+  __device__ DevBins(const PhaDev& p, const GridDev& g, int c, int e)
+      : n(p.nb),
+        hmw(p.hmw + (int64_t)c * p.nb * 2),
+        w((const O*)p.w + (int64_t)c * g.nE + e),
+        plane((int64_t)g.nC * g.nE) {}
+  __device__ void shift(int bin, D& hm, D& hw) const {
+    hm = (D)uload(hmw, 2 * bin);
+    hw = (D)uload(hmw, 2 * bin + 1);
+  }
+  __device__ O weight(int a, int bin) const { return w[((int64_t)a * n + bin) * plane]; }
+};
+
 // The atmosphere segment and the nine probabilities of the NC trajectories,
-// stored as rows of the [9][nC][nE] output.
-template <class P, int NC>
+// stored as rows of the [9][nC][nE] output (AVG: averaged over the
+// production-height bins of pha).
+template <class P, int NC, bool AVG = false>
 __device__ __forceinline__ void finish_store(const Traj<P> (&t)[NC], const Slab<P>& vac,
                                              const GridDev& g, const CosSet<NC>& cs,
                                              const SetConst& sc, void* outp, int e,
-                                             bool e_ok) {
+                                             bool e_ok, const PhaDev& pha) {
   using D = typename P::D;
   using F = typename P::F;
   using O = typename OutT<P>::type;
@@ -163,7 +186,13 @@ __device__ __forceinline__ void finish_store(const Traj<P> (&t)[NC], const Slab<
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     F prob[9];
-    traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cs.cidx[j]), sc, prob);
+    if constexpr (AVG) {
+      const DevBins<P> bins(pha, g, cs.cidx[j], e_ok ? e : g.nE - 1);
+      traj_finish_avg<P>(t[j], vac, (D)uload(g.Lvac, cs.cidx[j]), sc, bins, prob);
+    } else {
+      (void)pha;
+      traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cs.cidx[j]), sc, prob);
+    }
     if (e_ok && cs.ok[j]) {
       O* dst = out + (int64_t)cs.cidx[j] * g.nE + e;
 #pragma unroll
@@ -176,10 +205,10 @@ __device__ __forceinline__ void finish_store(const Traj<P> (&t)[NC], const Slab<
 // lane energy index e (stored only if e_ok), the lane's parameter set sc and
 // output array outp ([9][nC][nE]).  sc and outp are wave-uniform for a job's
 // own work units and per lane in packed tail tiles.
-template <class P, int NC, int W, class Stage>
+template <class P, int NC, int W, bool AVG = false, class Stage>
 __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, bool e_ok,
                                             const SetConst& sc, void* outp,
-                                            Stage& st) {
+                                            const PhaDev& pha, Stage& st) {
   using D = typename P::D;
   const int lane = (int)threadIdx.x % kWave;
   const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
@@ -260,18 +289,18 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
     __syncthreads();
   }
 
-  finish_store<P, NC>(t, vac, g, cs, sc, outp, e, e_ok);
+  finish_store<P, NC, AVG>(t, vac, g, cs, sc, outp, e, e_ok, pha);
 }
 
 // A work unit of one job: cosine block x one of its own 64-energy tiles.
-template <class P, int NC, int W, class Stage>
+template <class P, int NC, int W, bool AVG = false, class Stage>
 __device__ __forceinline__ void atmos_unit(const AtmosJob& job, int64_t local,
                                            Stage& st) {
   const int lane = (int)threadIdx.x % kWave;
   const int sb = (int)(local / job.etiles);
   const int et = (int)(local - (int64_t)sb * job.etiles);
   const int e = et * kWave + lane;
-  atmos_block<P, NC, W>(job.g, sb, e, e < job.g.nE, job.sc, job.out, st);
+  atmos_block<P, NC, W, AVG>(job.g, sb, e, e < job.g.nE, job.sc, job.out, job.pha, st);
 }
 
 // A work unit of a tail group: cosine block x packed tile t, whose lanes
@@ -289,7 +318,7 @@ __device__ __forceinline__ void atmos_tail_unit(const AtmosJob* jobs,
   const bool ok = q < tg.per && slot < tg.njobs;
   const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
   const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
-  atmos_block<P, NC, W>(tg.g, sb, e, ok, job.sc, job.out, st);
+  atmos_block<P, NC, W>(tg.g, sb, e, ok, job.sc, job.out, job.pha, st);
 }
 
 // Minimum waves per SIMD asked of each atmospheric variant, i.e. a cap of
@@ -442,10 +471,11 @@ __global__ void __launch_bounds__(kWave) slab_table_single_kernel(AtmosJob job,
 // The NC trajectories of cosine block sb (NC cosines) for one energy per
 // lane, reading slab row i of the lane's table column `col` (row stride
 // `row` entries), each row loaded one shell ahead of its use.
-template <class P, int NC>
+template <class P, int NC, bool AVG = false>
 __device__ __forceinline__ void table_block(const GridDev& g, int sb, int e, bool e_ok,
                                             const Slab<P>* col, int64_t row,
-                                            const SetConst& sc, void* outp) {
+                                            const SetConst& sc, void* outp,
+                                            const PhaDev& pha) {
   CosSet<NC> cs;
   cos_set<NC>(g, sb * NC, cs);
   Traj<P> t[NC];
@@ -465,12 +495,12 @@ __device__ __forceinline__ void table_block(const GridDev& g, int sb, int e, boo
     s = nx;
   }
   const Slab<P> vac = col[(int64_t)g.S * row];
-  finish_store<P, NC>(t, vac, g, cs, sc, outp, e, e_ok);
+  finish_store<P, NC, AVG>(t, vac, g, cs, sc, outp, e, e_ok, pha);
 }
 
 // A work unit of one job in table mode: energy tile x cosine block, the
 // cosine blocks of a tile consecutive so that they share its table rows.
-template <class P, int NC>
+template <class P, int NC, bool AVG = false>
 __device__ __forceinline__ void table_unit(const AtmosJob& job, int64_t local) {
   const int lane = (int)threadIdx.x;
   const int ncb = (int)cosine_blocks(job.g, NC, 1);
@@ -478,8 +508,8 @@ __device__ __forceinline__ void table_unit(const AtmosJob& job, int64_t local) {
   const int sb = (int)(local - (int64_t)et * ncb);
   const int e = et * kWave + lane;
   const int64_t row = (int64_t)energy_tiles(job.g) * kWave;
-  table_block<P, NC>(job.g, sb, e, e < job.g.nE, (const Slab<P>*)job.slabs + e, row,
-                     job.sc, job.out);
+  table_block<P, NC, AVG>(job.g, sb, e, e < job.g.nE, (const Slab<P>*)job.slabs + e, row,
+                          job.sc, job.out, job.pha);
 }
 
 // A packed tail unit in table mode: packed tile x cosine block.
@@ -496,7 +526,8 @@ __device__ __forceinline__ void tail_table_unit(const AtmosJob* jobs, const Tail
   const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
   const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
   const int64_t row = (int64_t)energy_tiles(tg.g) * kWave;
-  table_block<P, NC>(tg.g, sb, e, ok, (const Slab<P>*)job.slabs + e, row, job.sc, job.out);
+  table_block<P, NC>(tg.g, sb, e, ok, (const Slab<P>*)job.slabs + e, row, job.sc, job.out,
+                     job.pha);
 }
 
 // Minimum waves per SIMD of the table-mode kernels (0: natural allocation).
@@ -542,6 +573,26 @@ __global__ void __launch_bounds__(kWave, (MinWavesTable<P, NC>::value)) tail_tab
     }
     const TailGroup& tg = groups[lo];
     tail_table_unit<P, NC>(jobs, tg, u - tg.unit_begin);
+  }
+}
+
+// Production-height averaging on a single grid, one cosine per wave: the
+// in-kernel solve, one workgroup per unit ...
+template <class P>
+__global__ void __launch_bounds__(kWave) atmos_single_avg_kernel(AtmosJob job) {
+  NoStage st;
+  atmos_unit<P, 1, 1, true>(job, (int64_t)blockIdx.x, st);
+}
+
+// ... and the slab-table form, with the XCD-local walk of atmos_table_kernel.
+template <class P>
+__global__ void __launch_bounds__(kWave) atmos_table_avg_kernel(
+    const AtmosJob* __restrict__ job, int64_t nunits) {
+  const int64_t per = (nunits + 7) / 8;
+  for (int64_t b = blockIdx.x; b < per * 8; b += gridDim.x) {
+    const int64_t u = (b % 8) * per + b / 8;
+    if (u >= nunits) continue;
+    table_unit<P, 1, true>(*job, u);
   }
 }
 
@@ -710,6 +761,34 @@ hipError_t launch_slab_table(int precision, const AtmosJob* jobs, int njobs,
     case kFP64: slab_table_kernel<PolicyF64><<<g, b, 0, st>>>(jobs); break;
     case kFP32: slab_table_kernel<PolicyF32><<<g, b, 0, st>>>(jobs); break;
     case kMixed: slab_table_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_atmos_single_avg(int precision, const AtmosJob& job, hipStream_t st) {
+  const int64_t n = atmos_units(job.g, 1, 1);
+  if (n <= 0) return hipSuccess;
+  if (n > 0x7fffffff) return hipErrorInvalidConfiguration;
+  const dim3 g((unsigned)n), b(kWave);
+  switch (precision) {
+    case kFP64: atmos_single_avg_kernel<PolicyF64><<<g, b, 0, st>>>(job); break;
+    case kFP32: atmos_single_avg_kernel<PolicyF32><<<g, b, 0, st>>>(job); break;
+    case kMixed: atmos_single_avg_kernel<PolicyMixed><<<g, b, 0, st>>>(job); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_atmos_table_avg(int precision, const AtmosJob* d_job, int64_t nunits,
+                                  int grid, hipStream_t st) {
+  if (nunits <= 0) return hipSuccess;
+  if (grid <= 0 || grid % 8) return hipErrorInvalidConfiguration;
+  const dim3 g((unsigned)grid), b(kWave);
+  switch (precision) {
+    case kFP64: atmos_table_avg_kernel<PolicyF64><<<g, b, 0, st>>>(d_job, nunits); break;
+    case kFP32: atmos_table_avg_kernel<PolicyF32><<<g, b, 0, st>>>(d_job, nunits); break;
+    case kMixed: atmos_table_avg_kernel<PolicyMixed><<<g, b, 0, st>>>(d_job, nunits); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

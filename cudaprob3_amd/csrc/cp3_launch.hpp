@@ -28,6 +28,17 @@ struct GridDev {
   int32_t nC, nE, S, rye_cs;
 };
 
+// Production-height averaging of a single-grid job: the bin probabilities of
+// the job's neutrino type, [3][nb][nC][nE] (initial flavour, bin, cosine,
+// energy; element type of the output), and per cosine the bins' mid
+// path-length shifts and widths in km, [nC][nb][2].  Read only by the
+// averaging kernels.
+struct PhaDev {
+  const void* w;
+  const double* hmw;
+  int32_t nb, pad;
+};
+
 // One atmospheric evaluation: a parameter set and neutrino type on a grid.
 // Output layout [9][nC][nE] (channel a*3+b = P(a->b)), element type double
 // for kFP64 and float otherwise.
@@ -44,6 +55,7 @@ struct AtmosJob {
   // job, Slab<P> entries [g.S + 1][energy_tiles(g) * 64] (row g.S: the
   // vacuum).
   void* slabs;
+  PhaDev pha;
 };
 
 // Packed tail tiles: the last `tail` = nE % 64 energies of up to `per` =
@@ -123,6 +135,15 @@ hipError_t launch_tail_table(int precision, int nc, const AtmosJob* d_jobs,
                              const TailGroup* d_groups, int ngroups, int64_t nunits,
                              hipStream_t stream);
 hipError_t atmos_table_occupancy(int precision, int nc, int* blocks_per_cu);
+
+// Production-height averaging (job.pha) on a single grid, one cosine per
+// wave: the in-kernel solve (one workgroup per unit) or, after
+// launch_slab_table_single, the slab-table form reading the job at d_job
+// (grid a multiple of 8).
+hipError_t launch_atmos_single_avg(int precision, const AtmosJob& job,
+                                   hipStream_t stream);
+hipError_t launch_atmos_table_avg(int precision, const AtmosJob* d_job, int64_t nunits,
+                                  int grid, hipStream_t stream);
 
 // The packed tail tiles of a batch: one workgroup per unit.
 hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* d_jobs,

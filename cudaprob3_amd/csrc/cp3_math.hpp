@@ -596,6 +596,133 @@ CP3_HD void traj_finish(const Traj<P>& t, const Slab<P>& vac,
       prob[a * 3 + b] = ar[b][a] * ar[b][a] + ai[b][a] * ai[b][a];
 }
 
+// Production-height averaging (physics.hpp:997-1067, 1106-1130).  The
+// atmosphere segment splits the amplitude by vacuum eigenstate:
+// A = sum_k u_k v_k^T with u_k = e^{i psi_k} Z v_k (psi_0 = 0 for the
+// reference eigenvector, v_0 = v_a x v_b), so lengthening the path by s turns
+// component k by e^{2 pi i om_k s} (om_k in revolutions per km).  A bin of
+// path-length shifts [s0, s1] averages the interference of components j < k
+// into sinc(pi d hw) e^{2 pi i d hm} with d = om_j - om_k, hm = (s0 + s1) / 2
+// and hw = s1 - s0 (zero rates when the atmosphere segment has zero length,
+// physics.hpp:960-963), and the bins add up weighted by the bin probability
+// of the initial flavour a:
+//   P(a->b) = sum_k |q_k|^2 + 2 sum_{j<k} Re(q_j conj(q_k) X_jk(a)),
+//   q_k = U_k[b] V_k[a], U_k = R23 G u_k, V_k = R23 conj(G) v_k.
+// With every X equal to 1 this is |A[b][a]|^2.  The sinc is sin(x)/x for
+// |x| >= 1e-6 and 1 - x^2/6 + x^4/120 below (math.hpp:49-55, Epsilon() of
+// constants.hpp:49).  `bins` gives the bin count `n`, shift(bin, hm, hw) in km
+// and weight(a, bin).
+template <class P, class Bins>
+CP3_HD void traj_finish_avg(const Traj<P>& t, const Slab<P>& vac,
+                            typename P::D Lvac, const SetConst& sc,
+                            const Bins& bins, typename P::F prob[9]) {
+  using D = typename P::D;
+  using F = typename P::F;
+  const F zr[3][3] = {{t.y0.re[0], t.y0.re[1], t.y0.re[2]},
+                      {t.y0.re[1], t.y1.re[0], t.y1.re[1]},
+                      {t.y0.re[2], t.y1.re[1], t.y1.re[2]}};
+  const F zi[3][3] = {{t.y0.im[0], t.y0.im[1], t.y0.im[2]},
+                      {t.y0.im[1], t.y1.im[0], t.y1.im[1]},
+                      {t.y0.im[2], t.y1.im[1], t.y1.im[2]}};
+  F v[3][3];
+  for (int m = 0; m < 3; ++m) {
+    v[1][m] = vac.va[m];
+    v[2][m] = vac.vb[m];
+  }
+  for (int m = 0; m < 3; ++m) {
+    const int p = (m + 1) % 3, q = (m + 2) % 3;
+    v[0][m] = vac.va[p] * vac.vb[q] - vac.va[q] * vac.vb[p];
+  }
+  // u_k = c_k Z v_k, then U_k = R23 G u_k; V_k = R23 conj(G) v_k
+  const F cd = F(sc.cd), sd = F(sc.sd), c = F(sc.c23), s = F(sc.s23);
+  F ur[3][3], ui[3][3], vr[3][3], vi[3][3];
+  for (int k = 0; k < 3; ++k) {
+    F cr = F(1), ci = F(0);
+    if (k > 0) {
+      F wr, wi;
+      Phase<P>::eval(k == 1 ? vac.oma : vac.omb, Lvac, wr, wi);
+      cr = cr + wr;
+      ci = wi;
+    }
+    F tr[3], ti[3];
+    for (int i = 0; i < 3; ++i) {
+      const F xr = zr[i][0] * v[k][0] + zr[i][1] * v[k][1] + zr[i][2] * v[k][2];
+      const F xi = zi[i][0] * v[k][0] + zi[i][1] * v[k][1] + zi[i][2] * v[k][2];
+      tr[i] = cr * xr - ci * xi;
+      ti[i] = cr * xi + ci * xr;
+    }
+    const F g2r = cd * tr[2] - sd * ti[2], g2i = cd * ti[2] + sd * tr[2];
+    ur[k][0] = tr[0];
+    ui[k][0] = ti[0];
+    ur[k][1] = c * tr[1] + s * g2r;
+    ui[k][1] = c * ti[1] + s * g2i;
+    ur[k][2] = c * g2r - s * tr[1];
+    ui[k][2] = c * g2i - s * ti[1];
+    const F h2r = v[k][2] * cd, h2i = -(v[k][2] * sd);
+    vr[k][0] = v[k][0];
+    vi[k][0] = F(0);
+    vr[k][1] = c * v[k][1] + s * h2r;
+    vi[k][1] = s * h2i;
+    vr[k][2] = c * h2r - s * v[k][1];
+    vi[k][2] = c * h2i;
+  }
+  // pairs (0, 1), (0, 2), (1, 2)
+  const bool moving = Lvac != D(0);
+  const D om1 = moving ? vac.oma : D(0), om2 = moving ? vac.omb : D(0);
+  const D d[3] = {-om1, -om2, om1 - om2};
+  F xr[3][3], xi[3][3];  // [pair][initial flavour]
+  for (int p = 0; p < 3; ++p)
+    for (int a = 0; a < 3; ++a) {
+      xr[p][a] = F(0);
+      xi[p][a] = F(0);
+    }
+  for (int bin = 0; bin < bins.n; ++bin) {
+    D hm, hw;
+    bins.shift(bin, hm, hw);
+    F fr[3], fi[3];
+    for (int p = 0; p < 3; ++p) {
+      F er, ei;
+      Phase<P>::unit(d[p] * hm, er, ei);
+      const D q = D(0.5) * d[p] * hw;  // x = 2 pi q
+      const F x = F(D(2.0 * 3.14159265358979323846) * q);
+      F sinc;
+      if (m_fabs(x) >= F(1e-6)) {
+        F wr, sn;
+        m_cispi_m1(F(q - m_rint(q)), &wr, &sn);
+        sinc = sn / x;
+      } else {
+        sinc = F(1) - x * x / F(6) + x * x * x * x / F(120);
+      }
+      fr[p] = sinc * er;
+      fi[p] = sinc * ei;
+    }
+    for (int a = 0; a < 3; ++a) {
+      const F w = F(bins.weight(a, bin));
+      for (int p = 0; p < 3; ++p) {
+        xr[p][a] = xr[p][a] + w * fr[p];
+        xi[p][a] = xi[p][a] + w * fi[p];
+      }
+    }
+  }
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      F qr[3], qi[3];
+      F sum = F(0);
+      for (int k = 0; k < 3; ++k) {
+        qr[k] = ur[k][b] * vr[k][a] - ui[k][b] * vi[k][a];
+        qi[k] = ur[k][b] * vi[k][a] + ui[k][b] * vr[k][a];
+        sum = sum + (qr[k] * qr[k] + qi[k] * qi[k]);
+      }
+      for (int p = 0; p < 3; ++p) {
+        const int j = p < 2 ? 0 : 1, k = p == 0 ? 1 : 2;
+        const F pr = qr[j] * qr[k] + qi[j] * qi[k];
+        const F pi = qi[j] * qr[k] - qr[j] * qi[k];
+        sum = sum + F(2) * (pr * xr[p][a] - pi * xi[p][a]);
+      }
+      prob[a * 3 + b] = sum;
+    }
+}
+
 // A single constant-density layer of length L (the beam path,
 // physics.hpp:1153-1260): Z = I, then the same finish as a trajectory.
 template <class P>

@@ -45,6 +45,64 @@ void atmos_point(const cp3::SetConst& sc, const cp3::TrajTable& tt, int c,
   for (int k = 0; k < 9; ++k) out9[k] = (double)prob[k];
 }
 
+// Production-height bins of one point, read from host arrays: hmw of the
+// cosine ([nb][2], km) and the bin probabilities in the reference layout
+// [type][flavour][E][cos][bin] at this (type, E, cos).
+struct HostBins {
+  int n;
+  const double* hmw;
+  const double* w;
+  int64_t flav;
+  template <class D>
+  void shift(int b, D& hm, D& hw) const {
+    hm = (D)hmw[2 * b];
+    hw = (D)hmw[2 * b + 1];
+  }
+  double weight(int a, int b) const { return w[(int64_t)a * flav + b]; }
+};
+
+// The same point with production-height averaging.
+template <class P>
+void atmos_point_avg(const cp3::SetConst& sc, const cp3::TrajTable& tt, int c,
+                     double E, const HostBins& bins, double* out9) {
+  using D = typename P::D;
+  using F = typename P::F;
+  const D e = (D)E;
+  const D inv = D(1) / e;
+  const int M = tt.M[(size_t)c];
+  cp3::Traj<P> t;
+  cp3::traj_init<P>(t, M == 0);
+  for (int i = 0; i < M; ++i) {
+    cp3::Slab<P> s;
+    cp3::solve_slab<P>(sc, e, inv, (D)tt.rye[(size_t)tt.rye_cs * c + (size_t)i], s);
+    const D L = (D)tt.seg[(size_t)c * tt.S + (size_t)i];
+    if (i < M - 1)
+      cp3::traj_middle<P>(t, s, L);
+    else
+      cp3::traj_inner<P>(t, s, L);
+  }
+  cp3::Slab<P> vac;
+  cp3::solve_slab<P>(sc, e, inv, D(0), vac);
+  F prob[9];
+  cp3::traj_finish_avg<P>(t, vac, (D)tt.Lvac[(size_t)c], sc, bins, prob);
+  for (int k = 0; k < 9; ++k) out9[k] = (double)prob[k];
+}
+
+template <class P>
+void run_atmos_avg(const cp3::SetConst& sc, const cp3::TrajTable& tt, const double* ev,
+                   int nE, int nutype, int nb, const double* prob,
+                   const std::vector<double>& hmw, double* out) {
+  const int64_t nC = tt.nC;
+  const int64_t flav = (int64_t)nE * nC * nb;
+#pragma omp parallel for schedule(dynamic)
+  for (int c = 0; c < tt.nC; ++c)
+    for (int e = 0; e < nE; ++e) {
+      HostBins b{nb, hmw.data() + (size_t)c * nb * 2,
+                 prob + (int64_t)nutype * 3 * flav + ((int64_t)e * nC + c) * nb, flav};
+      atmos_point_avg<P>(sc, tt, c, ev[e], b, out + ((size_t)c * nE + e) * 9);
+    }
+}
+
 template <class P>
 void run_atmos(const cp3::SetConst& sc, const cp3::TrajTable& tt,
                const double* ev, int nE, double* out) {
@@ -304,6 +362,40 @@ int cp3m_atmos(int precision, int nutype, const double* cosv, int nC,
       run_atmos<cp3::PolicyF32>(sc, tt, ev, nE, out);
     else
       run_atmos<cp3::PolicyMixed>(sc, tt, ev, nE, out);
+  } catch (const std::exception& ex) {
+    put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
+// Atmospheric probabilities with production-height averaging: nb bins,
+// prob[2][3][nE][nC][nb] (setProductionHeightList's layout) and the nb + 1
+// bin edges in km.
+int cp3m_atmos_pha(int precision, int nutype, const double* cosv, int nC,
+                   const double* ev, int nE, const double* radii, const double* rhos,
+                   const double* yps, int nrows, double th12, double th13, double th23,
+                   double dcp, int kNuType, double dm12, double dm23, double h_km,
+                   int nb, const double* prob, const double* edges_km, double* out,
+                   char* err, int errlen) {
+  try {
+    cp3::EarthModel em;
+    cp3::earth_set(em, std::vector<double>(radii, radii + nrows),
+                   std::vector<double>(rhos, rhos + nrows),
+                   std::vector<double>(yps, yps + nrows));
+    cp3::TrajTable tt;
+    cp3::build_traj_table(tt, em, cosv, nC, h_km * 100000.0);
+    std::vector<double> hmw;
+    cp3::build_height_bins(hmw, cosv, nC, h_km * 100000.0,
+                           std::vector<double>(edges_km, edges_km + nb + 1));
+    const cp3::SetConst sc =
+        cp3::make_set(th12, th13, th23, dcp, kNuType, dm12, dm23, nutype);
+    if (precision == 0)
+      run_atmos_avg<cp3::PolicyF64>(sc, tt, ev, nE, nutype, nb, prob, hmw, out);
+    else if (precision == 1)
+      run_atmos_avg<cp3::PolicyF32>(sc, tt, ev, nE, nutype, nb, prob, hmw, out);
+    else
+      run_atmos_avg<cp3::PolicyMixed>(sc, tt, ev, nE, nutype, nb, prob, hmw, out);
   } catch (const std::exception& ex) {
     put_err(err, errlen, ex.what());
     return -1;

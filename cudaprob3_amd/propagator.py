@@ -123,6 +123,25 @@ class AtmosPropagator:
         N.check(self._lib.cp3_atmos_set_production_height(self._h,
                                                           float(height_km)))
 
+    def SetNumberOfProductionHeightBinsForAveraging(self, n_bins: int):
+        """At most 20 bins; like the reference (atmoscpupropagator.hpp:88-108)
+        any n >= 1 turns averaging off."""
+        N.check(self._lib.cp3_atmos_set_production_height_bins(self._h, int(n_bins)))
+
+    def setProductionHeightAveraging(self, on: bool = True):
+        """No counterpart in the reference class, which keeps this flag
+        protected (atmoscpupropagator.hpp:532) and never sets it: turns the
+        averaging of physics.hpp:997-1067 on or off."""
+        N.check(self._lib.cp3_atmos_set_production_height_averaging(self._h,
+                                                                    int(bool(on))))
+
+    def setProductionHeightList(self, list_prob, list_bins):
+        """Bin probabilities [nutype][flavour][energy][cosine][bin] and the
+        n_bins + 1 bin edges in km (atmoscpupropagator.hpp:395-427)."""
+        p, b = _f64(list_prob), _f64(list_bins)
+        N.check(self._lib.cp3_atmos_set_production_height_list(
+            self._h, _dptr(p), len(p), _dptr(b), len(b)))
+
     def getNlayerBoundaries(self) -> int:
         n = C.c_int()
         N.check(self._lib.cp3_atmos_get_nlayer_boundaries(self._h, C.byref(n)))

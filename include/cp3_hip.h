@@ -112,6 +112,30 @@ int cp3_atmos_set_production_height(cp3_atmos* p, double height_km);
 /* getNlayerBoundaries (atmoscpupropagator.hpp:85): number of model rows. */
 int cp3_atmos_get_nlayer_boundaries(cp3_atmos* p, int* n);
 
+/* SetNumberOfProductionHeightBinsForAveraging (atmoscpupropagator.hpp:88-108):
+ * 0..20 bins (CP3_ERR_INVALID above 20).  Like the reference, any n_bins >= 1
+ * turns production-height averaging off. */
+int cp3_atmos_set_production_height_bins(cp3_atmos* p, int n_bins);
+/* No counterpart in the reference class: turns production-height averaging on
+ * (on != 0) or off.  The reference class keeps this flag protected
+ * (atmoscpupropagator.hpp:532) and never sets it, while its physics core
+ * takes it as an argument (physics.hpp:804) and implements the averaging
+ * (physics.hpp:997-1067).  Either way a list set before is dropped. */
+int cp3_atmos_set_production_height_averaging(cp3_atmos* p, int on);
+/* setProductionHeightList (atmoscpupropagator.hpp:395-427): n_prob =
+ * n_bins * 2 * 3 * n_energies * n_cosines bin probabilities ordered
+ * [neutrino type][initial flavour][energy][cosine][bin] and n_edges =
+ * n_bins + 1 bin edges in km.  CP3_ERR_STATE unless averaging is on,
+ * CP3_ERR_INVALID on other sizes.  calculate then returns CP3_ERR_STATE while
+ * averaging is on without a list.  Each (cosine, energy) point averages over
+ * the path-length shifts of the bins, weighted per initial flavour; the sinc
+ * of a bin's width is sin(x)/x for |x| >= 1e-6 (the reference's CPU build
+ * truncates |x| to an integer in that test, math.hpp:50, and takes its series
+ * for every |x| < 1). */
+int cp3_atmos_set_production_height_list(cp3_atmos* p, const double* prob,
+                                         int64_t n_prob, const double* edges_km,
+                                         int n_edges);
+
 /* calculateProbabilities (atmoscudapropagator.cu:202-205, 266-316):
  * launches and waits. */
 int cp3_atmos_calculate(cp3_atmos* p, int nutype);

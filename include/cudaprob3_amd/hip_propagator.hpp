@@ -77,7 +77,7 @@ class HipPropagatorSingle {
   HipPropagatorSingle& operator=(const HipPropagatorSingle&) = delete;
   HipPropagatorSingle(HipPropagatorSingle&& o) noexcept
       : h_(o.h_), n_cosines_(o.n_cosines_), n_energies_(o.n_energies_),
-        n_prod_bins_(o.n_prod_bins_) {
+        avg_(o.avg_) {
     o.h_ = nullptr;
   }
   HipPropagatorSingle& operator=(HipPropagatorSingle&& o) noexcept {
@@ -86,7 +86,7 @@ class HipPropagatorSingle {
       h_ = o.h_;
       n_cosines_ = o.n_cosines_;
       n_energies_ = o.n_energies_;
-      n_prod_bins_ = o.n_prod_bins_;
+      avg_ = o.avg_;
       o.h_ = nullptr;
     }
     return *this;
@@ -159,20 +159,31 @@ class HipPropagatorSingle {
   void setProductionHeight(FLOAT_T heightKM) {
     check(cp3_atmos_set_production_height(live(), heightKM));
   }
-  // atmoscpupropagator.hpp:88-106: the reference clears the averaging flag
-  // for every n >= 1, so averaging never runs; same here.
+  // atmoscpupropagator.hpp:88-108: at most 20 bins; like the reference, any
+  // n >= 1 turns averaging off (setProductionHeightAveraging turns it on).
   void SetNumberOfProductionHeightBinsForAveraging(int n) {
-    if (n > 20)
-      throw std::runtime_error(
-          "SetNumberOfProductionHeightBinsForAveraging : invalid number of production height bins");
-    n_prod_bins_ = n;
-    std::cout << "Using fixed production height" << std::endl;
+    check(cp3_atmos_set_production_height_bins(live(), n));
+    if (n >= 1) avg_ = false;
+    if (avg_)
+      std::cout << "Set " << n << " Production height bins" << std::endl;
+    else
+      std::cout << "Using fixed production height" << std::endl;
   }
-  // atmoscpupropagator.hpp:395-398
-  void setProductionHeightList(const std::vector<FLOAT_T>&, const std::vector<FLOAT_T>&) {
-    throw std::runtime_error(
-        "Propagator::setProductionHeightList. Trying to set Production Height information "
-        "but propagator is not expecting to use it");
+  // No counterpart in the reference class, which keeps this flag protected
+  // (atmoscpupropagator.hpp:532) and never sets it: turns the averaging of
+  // physics.hpp:997-1067 on or off; a list set before is dropped.
+  void setProductionHeightAveraging(bool on) {
+    check(cp3_atmos_set_production_height_averaging(live(), on ? 1 : 0));
+    avg_ = on;
+  }
+  // atmoscpupropagator.hpp:395-427: bin probabilities
+  // [nutype][flavour][energy][cosine][bin] and the bin edges in km; throws
+  // unless averaging is on.
+  void setProductionHeightList(const std::vector<FLOAT_T>& list_prob,
+                               const std::vector<FLOAT_T>& list_bins) {
+    const std::vector<double> p = detail::to_double(list_prob), b = detail::to_double(list_bins);
+    check(cp3_atmos_set_production_height_list(live(), p.data(), (int64_t)p.size(), b.data(),
+                                               (int)b.size()));
   }
   // atmoscpupropagator.hpp:361-364: a placeholder in the reference too.
   void setPathLength(FLOAT_T) {
@@ -239,7 +250,8 @@ class HipPropagatorSingle {
     return h_;
   }
   cp3_atmos* h_ = nullptr;
-  int n_cosines_ = 0, n_energies_ = 0, n_prod_bins_ = 0;
+  int n_cosines_ = 0, n_energies_ = 0;
+  bool avg_ = false;
 };
 
 template <class FLOAT_T>

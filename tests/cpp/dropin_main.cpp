@@ -5,6 +5,7 @@
 //
 //   dropin_main atmos <out.bin> <precision 0|1|2> <nutype 0|1>
 //   dropin_main poly  <out.bin> <precision 0|1|2> <nutype 0|1>
+//   dropin_main avg   <out.bin> <precision 0|1|2> <nutype 0|1>
 //   dropin_main beam  <out.bin> <precision 0|1|2> <nutype 0|1>
 //   dropin_main errors
 //
@@ -12,6 +13,8 @@
 // log), nominal parameters, h = 22 km; output [cos][E][9] doubles.
 // poly: the same grid through setDensity(radii, a, b, c, yps) with the
 // quadratic profiles of cudaprob3_amd.workloads.prem4_poly.
+// avg: the atmos case with production-height averaging over 5 bins (edges 5,
+// 10, 15, 22, 30, 50 km), bin probabilities height_prob() below.
 // beam: 1300 km, 2.6 g/cm^3, 500 energies over 0.1-20 GeV; output [E][9].
 #include <cmath>
 #include <cstdio>
@@ -36,7 +39,28 @@ static const double kTh13 = std::asin(std::sqrt(0.0220));
 static const double kTh23 = std::asin(std::sqrt(0.546));
 static const double kDcp = -1.6, kDm12 = 7.53e-5, kDm23 = 2.494e-3;
 
-static int run_atmos(const char* path, int precision, int nutype, bool poly = false) {
+// Bin probabilities [nutype][flavour][E][cos][bin] for the avg case:
+// proportional to 1 + (7t + 5f + 3e + c + 11b) mod 13, normalised per
+// (t, f, e, c).
+static std::vector<double> height_prob(int nC, int nE, int nb) {
+  std::vector<double> v((size_t)2 * 3 * nE * nC * nb);
+  for (int t = 0; t < 2; ++t)
+    for (int f = 0; f < 3; ++f)
+      for (int e = 0; e < nE; ++e)
+        for (int c = 0; c < nC; ++c) {
+          double* w = &v[((((size_t)t * 3 + f) * nE + e) * nC + c) * nb];
+          double sum = 0;
+          for (int b = 0; b < nb; ++b) {
+            w[b] = 1.0 + (double)((7 * t + 5 * f + 3 * e + c + 11 * b) % 13);
+            sum += w[b];
+          }
+          for (int b = 0; b < nb; ++b) w[b] /= sum;
+        }
+  return v;
+}
+
+static int run_atmos(const char* path, int precision, int nutype, bool poly = false,
+                     bool avg = false) {
   const int nC = 100, nE = 100;
   std::vector<double> cz((size_t)nC);
   for (int i = 0; i < nC; ++i) cz[(size_t)i] = -1.0 + i / 100.0;
@@ -59,7 +83,12 @@ static int run_atmos(const char* path, int precision, int nutype, bool poly = fa
   }
   p.setProductionHeight(22.0);
   p.setMaxlayers();
-  if (poly) {
+  if (avg) {
+    p.SetNumberOfProductionHeightBinsForAveraging(5);
+    p.setProductionHeightAveraging(true);
+    p.setProductionHeightList(height_prob(nC, nE, 5), {5.0, 10.0, 15.0, 22.0, 30.0, 50.0});
+  }
+  if (poly || avg) {
     p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
   } else {
     // the CUDA class's asynchronous names (atmoscudapropagator.cuh:114-120)
@@ -151,6 +180,37 @@ static int run_errors() {
   bad += !throws([&] { p.getProbability(0, 4, e_e); }, "Invalid indices");
   bad += !throws([&] { p.setChemicalComposition({0.5}); }, "Size of input list");
   bad += !throws([&] { p.setProductionHeightList({1.0}, {1.0, 2.0}); }, "not expecting");
+  // production-height averaging, which the reference class never turns on
+  bad += !throws([&] { p.SetNumberOfProductionHeightBinsForAveraging(21); },
+                 "invalid number of production height bins");
+  std::vector<double> plain(3 * 4 * 9);
+  for (int c = 0; c < 3; ++c)
+    for (int e = 0; e < 4; ++e)
+      for (int t = 0; t < 9; ++t) plain[(c * 4 + e) * 9 + t] = p.getProbability(c, e, (ProbType)t);
+  p.SetNumberOfProductionHeightBinsForAveraging(1);
+  p.setProductionHeightAveraging(true);
+  bad += !throws([&] { p.calculateProbabilities(Neutrino); }, "production height array was not set");
+  bad += !throws([&] { p.setProductionHeightList({1.0}, {15.0, 15.0}); },
+                 "Prob array is not the expected size");
+  bad += !throws([&] { p.setProductionHeightList(std::vector<double>(72, 1.0), {15.0}); },
+                 "ProductionHeightBins array is not expected size");
+  // one zero-width bin at the production height: the unaveraged result
+  p.setProductionHeightList(std::vector<double>(72, 1.0), {15.0, 15.0});
+  p.calculateProbabilities(Neutrino);
+  double dev = 0;
+  for (int c = 0; c < 3; ++c)
+    for (int e = 0; e < 4; ++e)
+      for (int t = 0; t < 9; ++t)
+        dev = std::fmax(dev, std::fabs(p.getProbability(c, e, (ProbType)t) -
+                                       plain[(c * 4 + e) * 9 + t]));
+  if (dev > 1e-12) {
+    std::fprintf(stderr, "zero-width bin moved results by %.3g\n", dev);
+    ++bad;
+  }
+  // n >= 1 turns averaging off again, as in the reference
+  p.SetNumberOfProductionHeightBinsForAveraging(2);
+  bad += !throws([&] { p.setProductionHeightList(std::vector<double>(144, 0.5), {1.0, 2.0, 3.0}); },
+                 "not expecting");
   HipPropagatorSingle<double> q(std::move(p));
   bad += !throws([&] { p.calculateProbabilities(Neutrino); }, "moved from");
   q.calculateProbabilities(Antineutrino);
@@ -172,13 +232,15 @@ int main(int argc, char** argv) {
   try {
     if (argc >= 2 && std::string(argv[1]) == "errors") return run_errors();
     if (argc < 5) {
-      std::fprintf(stderr, "usage: %s atmos|poly|beam out.bin precision nutype | errors\n", argv[0]);
+      std::fprintf(stderr, "usage: %s atmos|poly|avg|beam out.bin precision nutype | errors\n",
+                   argv[0]);
       return 2;
     }
     const std::string mode = argv[1];
     const int prec = std::atoi(argv[3]), nt = std::atoi(argv[4]);
     if (mode == "atmos") return run_atmos(argv[2], prec, nt);
     if (mode == "poly") return run_atmos(argv[2], prec, nt, true);
+    if (mode == "avg") return run_atmos(argv[2], prec, nt, false, true);
     if (mode == "beam") return run_beam(argv[2], prec, nt);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "exception: %s\n", e.what());

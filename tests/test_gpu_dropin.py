@@ -62,6 +62,31 @@ def test_dropin_atmos_poly_fp64(oracle, tmp_path, nutype):
     assert np.abs(got - ref).max() < 1e-10
 
 
+def dropin_height_prob(nC, nE, nb):
+    """The bin probabilities of dropin_main's avg case."""
+    t, f, e, c, b = np.meshgrid(np.arange(2), np.arange(3), np.arange(nE),
+                                np.arange(nC), np.arange(nb), indexing="ij")
+    w = 1.0 + ((7 * t + 5 * f + 3 * e + c + 11 * b) % 13)
+    return w / w.sum(axis=-1, keepdims=True)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (2, 1e-5)])
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_dropin_height_avg(oracle, tmp_path, nutype, prec, tol):
+    """Production-height averaging through the C++ class
+    (setProductionHeightAveraging, then the reference's
+    SetNumberOfProductionHeightBinsForAveraging / setProductionHeightList)."""
+    out = tmp_path / "h.bin"
+    assert run("avg", out, prec, nutype).returncode == 0
+    got = np.fromfile(out, dtype=np.float64).reshape(100, 100, 9)
+    e = W.prem4()
+    ref = oracle.atmos_pha(nutype, W.cos_grid(100), W.log_grid(100, 0.1, 100.0),
+                           e.radii, e.rhos, e.yps, osc_for(NOM, nutype), 22.0,
+                           dropin_height_prob(100, 100, 5),
+                           [5.0, 10.0, 15.0, 22.0, 30.0, 50.0], threads=4)
+    assert np.abs(got - ref).max() < tol
+
+
 @pytest.mark.parametrize("nutype", [0, 1])
 def test_dropin_beam_fp64(oracle, tmp_path, nutype):
     out = tmp_path / "b.bin"

@@ -287,3 +287,146 @@ def test_wide_parameter_sweep(oracle):
     print("wide sweep: fp64", worst64, "mixed", worstm)
     assert worst64 < TOL_FP64
     assert worstm < TOL_MIXED
+
+
+# ------------------------------------------- production-height averaging
+
+def run_atmos_avg(cz, ev, earth, params, nutype, prob, edges, precision="fp64",
+                  h_km=22.0, mode="auto"):
+    from cudaprob3_amd import AtmosPropagator
+    with AtmosPropagator(len(cz), len(ev), precision=precision) as p:
+        p.setSolveMode(mode)
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        t12, t13, t23, d, m12, m23 = [float(x) for x in params]
+        p.setMNSMatrix(t12, t13, t23, d, -1 if nutype else 1)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(earth.radii, earth.rhos, earth.yps)
+        p.setProductionHeight(h_km)
+        p.SetNumberOfProductionHeightBinsForAveraging(len(edges) - 1)
+        p.setProductionHeightAveraging(True)
+        p.setProductionHeightList(np.asarray(prob).ravel(), edges)
+        p.calculateProbabilities(nutype)
+        return p.probabilities()
+
+
+def row_unitarity(prob):
+    """Each initial flavour's probabilities sum to 1; with bin weights that
+    differ between initial flavours the columns need not."""
+    return np.abs(prob.reshape(-1, 3, 3).sum(-1) - 1).max()
+
+
+def height_prob(nC, nE, nb, seed):
+    pr = np.random.default_rng(seed).random((2, 3, nE, nC, nb))
+    return pr / pr.sum(axis=-1, keepdims=True)
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_height_avg_fp64_vs_golden(oracle, nutype):
+    """Production-height averaging on the GPU against the golden made by the
+    compiled reference's averaging path: within 1e-10 of the oracle with the
+    intended sinc, and within 1e-4 of the golden (the reference CPU build's
+    integer abs() in its sinc test, math.hpp:50, takes the series for every
+    |x| < 1)."""
+    g = golden("g7_prem4_height_avg")
+    e = W.EarthModel(g["radii"], g["rhos"], g["yps"])
+    for mode in ("table", "shared"):
+        out = run_atmos_avg(g["cosv"], g["ev"], e, NOM, nutype, g["height_prob"],
+                            g["edges_km"], mode=mode)
+        ref = oracle.atmos_pha(nutype, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                               g["yps"], osc_for(NOM, nutype), 22.0,
+                               g["height_prob"], g["edges_km"])
+        print("height avg", mode, nutype, "max|d| vs oracle", np.abs(out - ref).max(),
+              "vs golden", np.abs(out - g["prob"][nutype]).max())
+        assert np.abs(out - ref).max() < TOL_FP64
+        assert np.abs(out - g["prob"][nutype]).max() < 1e-4
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("mixed", TOL_MIXED),
+                                           ("fp32", TOL_FP32)])
+def test_height_avg_vs_oracle(oracle, precision, tol):
+    """12 shells, 61 x 70 points (a ragged last energy tile), 7 bins from 0.5
+    to 60 km, drawn parameter sets, both neutrino types; the slab-table and
+    in-kernel solves give the same bits."""
+    e = W.earth_12()
+    cz = np.linspace(-1.0, 1.0, 61)
+    ev = W.log_grid(70, 0.1, 100.0)
+    edges = np.array([0.5, 4.0, 9.0, 15.0, 22.0, 30.0, 41.0, 60.0])
+    pr = height_prob(len(cz), len(ev), len(edges) - 1, 8)
+    for k, p in enumerate(W.draw_param_sets(2, 88)):
+        row = [p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+        for nt in (0, 1):
+            ref = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps,
+                                   osc_for(row, nt), 18.0, pr, edges, threads=4)
+            a = run_atmos_avg(cz, ev, e, row, nt, pr, edges, precision, 18.0, "table")
+            b = run_atmos_avg(cz, ev, e, row, nt, pr, edges, precision, 18.0, "shared")
+            print(precision, k, nt, "max|d|", np.abs(a - ref).max())
+            assert np.abs(a - ref).max() < tol
+            assert np.array_equal(a, b)
+            assert row_unitarity(a) < {"fp64": 1e-12, "mixed": TOL_MIXED, "fp32": TOL_FP32}[precision]
+
+
+def test_height_avg_limits():
+    """A zero-width bin at the production height gives the unaveraged
+    result, also through a polynomial model (per-trajectory solves); a
+    propagator switched back off runs the plain path again."""
+    cz = np.linspace(-1.0, 0.95, 40)
+    ev = W.log_grid(90, 0.1, 50.0)
+    one = np.ones((2, 3, len(ev), len(cz), 1))
+    e = W.prem4()
+    for nt in (0, 1):
+        plain = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, nt)
+        avg = run_atmos_avg(cz, ev, e, NOM, nt, one, [22.0, 22.0])
+        assert np.abs(avg - plain).max() < 1e-12
+    ep = W.prem4_poly()
+    from cudaprob3_amd import AtmosPropagator
+    with AtmosPropagator(len(cz), len(ev)) as p:
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        p.setMNSMatrix(W.TH12, W.TH13, W.TH23, W.DCP, 1)
+        p.setNeutrinoMasses(W.DM12, W.DM23)
+        p.setDensity(ep.radii, ep.a, ep.b, ep.c, ep.yps)
+        p.setProductionHeight(22.0)
+        p.calculateProbabilities(0)
+        plain = p.probabilities()
+        p.SetNumberOfProductionHeightBinsForAveraging(1)
+        p.setProductionHeightAveraging(True)
+        p.setProductionHeightList(one.ravel(), [22.0, 22.0])
+        p.calculateProbabilities(0)
+        assert np.abs(p.probabilities() - plain).max() < 1e-12
+        p.setProductionHeightAveraging(False)
+        p.calculateProbabilities(0)
+        assert np.array_equal(p.probabilities(), plain)
+
+
+def test_height_avg_errors():
+    """The reference's call-order and size rules (atmoscpupropagator.hpp:88-108,
+    395-427, 453-454), as Cp3Error."""
+    from cudaprob3_amd import AtmosPropagator
+    from cudaprob3_amd._native import Cp3Error
+    with AtmosPropagator(3, 4) as p:
+        p.setEnergyList([1.0, 2.0, 3.0, 4.0])
+        p.setCosineList([-1.0, -0.5, 0.2])
+        p.setMNSMatrix(W.TH12, W.TH13, W.TH23, W.DCP, 1)
+        p.setNeutrinoMasses(W.DM12, W.DM23)
+        p.setDensity([0.0, 3480.0, 6371.0], [11.3, 11.3, 4.0], [0.5, 0.5, 0.5])
+        p.setProductionHeight(15.0)
+        with pytest.raises(Cp3Error, match="invalid number of production height bins"):
+            p.SetNumberOfProductionHeightBinsForAveraging(21)
+        with pytest.raises(Cp3Error, match="not expecting"):
+            p.setProductionHeightList(np.ones(72), [1.0, 2.0])
+        p.SetNumberOfProductionHeightBinsForAveraging(1)
+        p.setProductionHeightAveraging(True)
+        with pytest.raises(Cp3Error, match="production height array was not set"):
+            p.calculateProbabilities(0)
+        with pytest.raises(Cp3Error, match="Prob array is not the expected size"):
+            p.setProductionHeightList(np.ones(71), [1.0, 2.0])
+        with pytest.raises(Cp3Error, match="ProductionHeightBins array"):
+            p.setProductionHeightList(np.ones(72), [1.0, 2.0, 3.0])
+        # 1 bin x 2 types x 3 flavours x 4 energies x 3 cosines
+        p.setProductionHeightList(np.full(72, 1.0), [10.0, 20.0])
+        p.calculateProbabilities(1)
+        assert unitarity(p.probabilities()) < 1e-12
+        p.SetNumberOfProductionHeightBinsForAveraging(2)  # turns it off again
+        with pytest.raises(Cp3Error, match="not expecting"):
+            p.setProductionHeightList(np.ones(144), [1.0, 2.0, 3.0])

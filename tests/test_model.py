@@ -28,6 +28,10 @@ def model():
                                   [dp, C.c_char_p, C.c_int])
     m.cp3m_poly_rye.argtypes = ([dp, C.c_int] + [dp] * 5 +
                                 [C.c_int, C.c_double, dp])
+    m.cp3m_atmos_pha.argtypes = ([C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, dp,
+                                  dp, dp, C.c_int] + [C.c_double] * 4 +
+                                 [C.c_int] + [C.c_double] * 3 +
+                                 [C.c_int, dp, dp, dp, C.c_char_p, C.c_int])
     m.cp3m_beam.argtypes = ([C.c_int, C.c_int, dp, C.c_int, C.c_double,
                              C.c_double] + [C.c_double] * 4 + [C.c_int] +
                             [C.c_double] * 2 + [dp, C.c_char_p, C.c_int])
@@ -181,6 +185,29 @@ def test_poly_trajectory_densities_match_oracle(model, oracle):
     assert np.array_equal(out, ref)
 
 
+def test_poly_grazing_trajectories_are_finite(model, oracle):
+    """Just below the horizon the reference's segment average of a
+    polynomial model reaches 0/0 (its density, and every probability, is NaN;
+    the oracle restates that).  The product takes the limit of the mean over
+    a vanishing segment, the density at the shell's outer radius, and its
+    probabilities stay unitary."""
+    ep = W.prem4_poly()
+    cz = np.array([-1.1102230246251565e-16, -1e-12, -1e-9])
+    out = np.zeros((len(cz), len(ep.radii)))
+    assert model.cp3m_poly_rye(P(cz), len(cz), P(ep.radii), P(ep.a), P(ep.b),
+                               P(ep.c), P(ep.yps), len(ep.radii), 22.0,
+                               out.ctypes.data_as(dp)) == 0
+    ref = oracle.poly_rye(cz, ep.radii, ep.a, ep.b, ep.c, ep.yps, 22.0)
+    assert np.isnan(ref[:, 0]).all()
+    R = ep.radii[-1]
+    rho_outer = (ep.a[-1] * R * R + ep.b[-1] * R + ep.c[-1]) * ep.yps[-1]
+    assert np.allclose(out[:, 0], rho_outer, rtol=1e-14, atol=0)
+    assert np.array_equal(out[:, 1:], ref[:, 1:])
+    probs = m_atmos_poly(model, 0, 0, cz, W.log_grid(8, 0.1, 50.0), ep, NOM)
+    assert np.isfinite(probs).all()
+    assert np.abs(probs.reshape(-1, 3, 3).sum(-1) - 1).max() < 1e-12
+
+
 @pytest.mark.parametrize("nutype", [0, 1])
 def test_poly_rewrite_vs_golden_and_oracle(model, oracle, nutype):
     """The kernel arithmetic through a polynomial model: FP64 within 1e-10 of
@@ -259,3 +286,82 @@ def test_density_file_formats(model, tmp_path):
     p4.write_text("0 1 2 3\n6371 1 2 3\n")
     rc, _, _, msg = _load_file(model, p4)
     assert rc == -1 and "4 entries per line" in msg
+
+
+# ------------------------------------------- production-height averaging
+
+def m_atmos_pha(m, prec, nt, cz, ev, earth, row, h, prob, edges):
+    cz = np.ascontiguousarray(cz, float)
+    ev = np.ascontiguousarray(ev, float)
+    prob = np.ascontiguousarray(prob, float)
+    edges = np.ascontiguousarray(edges, float)
+    out = np.zeros((len(cz), len(ev), 9))
+    err = C.create_string_buffer(256)
+    t12, t13, t23, d, m12, m23 = [float(x) for x in row]
+    rc = m.cp3m_atmos_pha(prec, nt, P(cz), len(cz), P(ev), len(ev),
+                          P(earth.radii), P(earth.rhos), P(earth.yps),
+                          len(earth.radii), t12, t13, t23, d, -1 if nt else 1,
+                          m12, m23, h, len(edges) - 1, P(prob), P(edges),
+                          out.ctypes.data_as(dp), err, 256)
+    assert rc == 0, err.value
+    return out
+
+
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_height_avg_rewrite_vs_oracle(model, oracle, nutype):
+    """Production-height averaging through the kernel arithmetic (the
+    amplitude split by vacuum eigenstate in the rotated frame) against the
+    oracle's restatement of the reference's expansion: FP64 within 1e-10,
+    mixed within 1e-5, FP32 within 1e-3; on the golden's inputs too, where
+    the difference from the golden is the reference CPU build's sinc."""
+    e = W.earth_12()
+    cz = np.linspace(-1.0, 1.0, 41)
+    ev = W.log_grid(37, 0.1, 100.0)
+    edges = np.array([0.5, 4.0, 9.0, 15.0, 22.0, 30.0, 41.0, 60.0])
+    pr = np.random.default_rng(5).random((2, 3, len(ev), len(cz), len(edges) - 1))
+    pr /= pr.sum(axis=-1, keepdims=True)
+    for k, row in enumerate([NOM] + [[p.th12, p.th13, p.th23, p.dcp, p.dm12, p.dm23]
+                                     for p in W.draw_param_sets(2, 31)]):
+        ref = oracle.atmos_pha(nutype, cz, ev, e.radii, e.rhos, e.yps,
+                               osc_for(row, nutype), 18.0, pr, edges, threads=4)
+        out = m_atmos_pha(model, 0, nutype, cz, ev, e, row, 18.0, pr, edges)
+        assert np.abs(out - ref).max() < 1e-10, k
+        mixed = m_atmos_pha(model, 2, nutype, cz, ev, e, row, 18.0, pr, edges)
+        assert np.abs(mixed - ref).max() < 1e-5, k
+        f32 = m_atmos_pha(model, 1, nutype, cz, ev, e, row, 18.0, pr, edges)
+        assert np.abs(f32 - ref).max() < 1e-3, k
+    g = golden("g7_prem4_height_avg")
+    e4 = W.EarthModel(g["radii"], g["rhos"], g["yps"])
+    out = m_atmos_pha(model, 0, nutype, g["cosv"], g["ev"], e4, NOM, 22.0,
+                      g["height_prob"], g["edges_km"])
+    fixed = oracle.atmos_pha(nutype, g["cosv"], g["ev"], g["radii"], g["rhos"],
+                             g["yps"], osc_for(NOM, nutype), 22.0,
+                             g["height_prob"], g["edges_km"])
+    assert np.abs(out - fixed).max() < 1e-10
+    assert np.abs(out - g["prob"][nutype]).max() < 1e-4
+
+
+def test_height_avg_limits_in_the_rewrite(model, oracle):
+    """The split reassembles the plain amplitude: a zero-width bin at the
+    production height itself gives the unaveraged result, no bins give the
+    incoherent sum, and a zero-length atmosphere segment (cos = -1 at height
+    0) turns no phase."""
+    e = W.prem4()
+    cz = np.linspace(-1.0, 1.0, 17)
+    ev = W.log_grid(13, 0.1, 50.0)
+    one = np.ones((2, 3, len(ev), len(cz), 1))
+    for nt in (0, 1):
+        plain = m_atmos(model, 0, nt, cz, ev, e, NOM, h=22.0)
+        avg = m_atmos_pha(model, 0, nt, cz, ev, e, NOM, 22.0, one, [22.0, 22.0])
+        assert np.abs(avg - plain).max() < 1e-12
+        none = np.zeros((2, 3, len(ev), len(cz), 0))
+        inc = m_atmos_pha(model, 0, nt, cz, ev, e, NOM, 22.0, none, [22.0])
+        ref = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(NOM, nt),
+                               22.0, none, [22.0])
+        assert np.abs(inc - ref).max() < 1e-12
+        z = m_atmos_pha(model, 0, nt, [-1.0], ev, e, NOM, 0.0,
+                        np.ones((2, 3, len(ev), 1, 1)), [0.0, 30.0])
+        zr = oracle.atmos_pha(nt, [-1.0], ev, e.radii, e.rhos, e.yps,
+                              osc_for(NOM, nt), 0.0, np.ones((2, 3, len(ev), 1, 1)),
+                              [0.0, 30.0])
+        assert np.abs(z - zr).max() < 1e-10
