@@ -19,14 +19,19 @@
 //   * polynomial models need b and c lists as long as the radii (the
 //     reference checks only a's size, atmoscpupropagator.hpp:190-200, and
 //     would read past a shorter list), and the cos = -1 branch of the
-//     polynomial average reads radius 0 past a model without its 0-km row.
+//     polynomial average reads radius 0 past a model without its 0-km row;
+//   * production-height averaging's sinc (math.hpp:49-55) is the intended
+//     sin(A)/A for |A| >= 1e-6 unless `int_abs_sinc` asks for the compiled
+//     reference's form, whose abs() truncates to an integer.
 //
 // Pins (tests/): golden vectors made by the compiled reference
 // (tests/golden/), the reference library itself where it is built
 // (oracle/_ref/), closed forms (vacuum oscillation, U.U^+ = I, eigenvalues vs
 // numpy.linalg.eigvalsh, one layer vs scipy.linalg.expm, chord geometry sums)
 // and invariants (unitarity, nu-bar <-> (delta, A) -> (-delta, -A), T-symmetry
-// at delta = 0).
+// at delta = 0); production-height averaging against zero-width bins (the
+// plain propagation from that height), Gauss-Legendre means over path-length
+// shifts and the incoherent limit.
 
 #include <algorithm>
 #include <cmath>
