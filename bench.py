@@ -92,8 +92,10 @@ def time_steps(b, steps, warmup, graph):
 def other_configs():
     """Kernel time of the other throughput configurations, median of 5 runs
     after one warm-up: the single-grid propagator on the 12-shell model at
-    2000 x 2000 (0.1-100 GeV, nominal parameters, neutrinos; FP64, mixed, FP32)
-    and the beam batch of 4096 seeded sets x 1e5 energies (0.1-20 GeV, 1300 km,
+    2000 x 2000 (0.1-100 GeV, nominal parameters, neutrinos; FP64, mixed, FP32),
+    the same grid with production-height averaging over 20 bins (edges 5-50 km,
+    equal bin probabilities; FP64, mixed; pre-pass plus averaging kernel), and
+    the beam batch of 4096 seeded sets x 1e5 energies (0.1-20 GeV, 1300 km,
     2.6 g/cm^3) in one mixed launch."""
     from cudaprob3_amd import AtmosPropagator
     from cudaprob3_amd.batch import BEAM
@@ -118,6 +120,29 @@ def other_configs():
             t = float(np.median(ms[1:]))
             single[prec] = {"kernel_ms": t, "evals_per_s": n * n / (t / 1e3)}
     out["single_grid_12shell_2000x2000"] = single
+    nb = 20
+    prob = np.full(2 * 3 * n * n * nb, 1.0 / nb)
+    edges = np.linspace(5.0, 50.0, nb + 1)
+    avg = {}
+    for prec in ("fp64", "mixed"):
+        with AtmosPropagator(n, n, precision=prec) as p:
+            p.setEnergyList(ev)
+            p.setCosineList(cz)
+            p.setMNSMatrix(t12, t13, t23, d, k)
+            p.setNeutrinoMasses(m12, m23)
+            p.setDensity(e.radii, e.rhos, e.yps)
+            p.setProductionHeight(W.PROD_HEIGHT_KM)
+            p.SetNumberOfProductionHeightBinsForAveraging(nb)
+            p.setProductionHeightAveraging(True)
+            p.setProductionHeightList(prob, edges)
+            ms = []
+            for _ in range(6):
+                p.calculateProbabilities(0)
+                ms.append(p.lastKernelMs())
+            t = float(np.median(ms[1:]))
+            avg[prec] = {"kernel_ms": t, "evals_per_s": n * n / (t / 1e3)}
+    del prob
+    out["height_avg_12shell_2000x2000_20bins"] = avg
     bev = W.log_grid(100000, 0.1, 20.0)
     sets = W.draw_param_sets(4096, 4096)
     with Batch("mixed") as b:
@@ -220,6 +245,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the single-grid (12 shells, 2000 x 2000) and beam "
                          "(4096 sets x 1e5 energies) timings")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="print only those timings (one JSON line) and exit")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -237,6 +264,9 @@ def main():
 
     if args.calib_copy_mib:
         calib_copy(args.calib_copy_mib << 20)
+    if args.configs_only:
+        print(json.dumps({"configs": other_configs()}))
+        return
 
     wl = W.atmos1024(n_sets=args.sets)
     evals = wl.n_evals
