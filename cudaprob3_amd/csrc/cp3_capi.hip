@@ -126,6 +126,10 @@ struct cp3_atmos {
   }
 
   void upload_tables() {
+    // the uploads are synchronous copies outside this object's stream, which
+    // an asynchronous calculation may still be reading
+    if (launched && (energies_dirty || table_dirty))
+      hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (energies_dirty) {
       d_ev.upload(energies.data(), energies.size() * sizeof(double), dev);
       energies_dirty = false;
@@ -283,6 +287,8 @@ struct cp3_beam {
   void launch(int nutype) {
     DeviceScope ds(dev);
     if (energies_dirty) {
+      // an asynchronous calculation may still read the old list
+      if (launched) hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
       d_ev.upload(energies.data(), energies.size() * sizeof(double), dev);
       energies_dirty = false;
     }
