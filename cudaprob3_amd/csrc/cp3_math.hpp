@@ -184,24 +184,24 @@ CP3_HD void m_sincos(float x, float* s, float* c) {
 // come from fit_cispi.py), then w = (cos 2 pi f - 1, sin 2 pi f)
 // = (-2 sin^2, 2 sin cos)(pi f) = (-a^2, a b), free of cancellation for
 // small f.
-CP3_HD void m_cispi_m1(float f, float* wr, float* wi) {
+// The two polynomials: pa = sqrt(2) sin(pi f) / f and b = sqrt(2) cos(pi f).
+CP3_HD void m_half_sc(float f, float* pa_out, float* b_out) {
   const float z = f * f;
   float pa = 0.10920575261116028f;
   pa = pa * z + -0.845763623714447f;
   pa = pa * z + 3.6062889099121094f;
   pa = pa * z + -7.308241367340088f;
   pa = pa * z + 4.442883014678955f;
-  const float a = pa * f;
   float b = -0.034502167254686356f;
   b = b * z + 0.3322518765926361f;
   b = b * z + -1.8882750272750854f;
   b = b * z + 5.73988151550293f;
   b = b * z + -6.9788641929626465f;
   b = b * z + 1.4142135381698608f;
-  *wr = -(a * a);
-  *wi = a * b;
+  *pa_out = pa;
+  *b_out = b;
 }
-CP3_HD void m_cispi_m1(double f, double* wr, double* wi) {
+CP3_HD void m_half_sc(double f, double* pa_out, double* b_out) {
   const double z = f * f;
   double pa = -2.9819668671076778e-05;
   pa = fma(pa, z, 0.0006589415933150775);
@@ -211,7 +211,6 @@ CP3_HD void m_cispi_m1(double f, double* wr, double* wi) {
   pa = fma(pa, z, 3.606476571424355);
   pa = fma(pa, z, -7.308249499994612);
   pa = fma(pa, z, 4.442882938158362);
-  const double a = pa * f;
   double b = 5.8684089087724815e-06;
   b = fma(b, z, -0.00014787947669297268);
   b = fma(b, z, 0.0027288046656365815);
@@ -221,8 +220,26 @@ CP3_HD void m_cispi_m1(double f, double* wr, double* wi) {
   b = fma(b, z, 5.739885734946355);
   b = fma(b, z, -6.978864199638876);
   b = fma(b, z, 1.4142135623730951);
+  *pa_out = pa;
+  *b_out = b;
+}
+template <class T>
+CP3_HD void m_cispi_m1(T f, T* wr, T* wi) {
+  T pa, b;
+  m_half_sc(f, &pa, &b);
+  const T a = pa * f;
   *wr = -(a * a);
   *wi = a * b;
+}
+
+// sin(2 pi q) / (2 pi q) for |q| <= 1/2, from the same polynomials with no
+// division: sin(2 pi q) = a b = pa q b, so the quotient is pa b / (2 pi)
+// (exactly 1 at q = 0 up to the polynomials' rounding).
+template <class T>
+CP3_HD T m_sinc2pi_half(T q) {
+  T pa, b;
+  m_half_sc(q, &pa, &b);
+  return pa * b * T(kInvTwoPi);
 }
 
 // Layer phases in revolutions: om is in revolutions per km, the phase
@@ -679,22 +696,29 @@ CP3_HD void traj_finish_avg(const Traj<P>& t, const Slab<P>& vac,
   for (int bin = 0; bin < bins.n; ++bin) {
     D hm, hw;
     bins.shift(bin, hm, hw);
+    // e^{2 pi i d hm} per pair; the third is the first two's quotient
+    F er[3], ei[3];
+    Phase<P>::unit(d[0] * hm, er[0], ei[0]);
+    Phase<P>::unit(d[1] * hm, er[1], ei[1]);
+    er[2] = er[1] * er[0] + ei[1] * ei[0];
+    ei[2] = ei[1] * er[0] - er[1] * ei[0];
     F fr[3], fi[3];
     for (int p = 0; p < 3; ++p) {
-      F er, ei;
-      Phase<P>::unit(d[p] * hm, er, ei);
-      const D q = D(0.5) * d[p] * hw;  // x = 2 pi q
-      const F x = F(D(2.0 * 3.14159265358979323846) * q);
+      // sinc(x), x = 2 pi q: a widths' phase is below half a revolution in
+      // practice (polynomial quotient); wider bins divide the reduced sine
+      const D q = D(0.5) * d[p] * hw;
       F sinc;
-      if (m_fabs(x) >= F(1e-6)) {
+      if (m_fabs(q) <= D(0.5)) {
+        const F x = F(D(2.0 * 3.14159265358979323846) * q);
+        sinc = m_fabs(x) >= F(1e-6) ? m_sinc2pi_half(F(q))
+                                    : F(1) - x * x / F(6) + x * x * x * x / F(120);
+      } else {
         F wr, sn;
         m_cispi_m1(F(q - m_rint(q)), &wr, &sn);
-        sinc = sn / x;
-      } else {
-        sinc = F(1) - x * x / F(6) + x * x * x * x / F(120);
+        sinc = sn / F(D(2.0 * 3.14159265358979323846) * q);
       }
-      fr[p] = sinc * er;
-      fi[p] = sinc * ei;
+      fr[p] = sinc * er[p];
+      fi[p] = sinc * ei[p];
     }
     for (int a = 0; a < 3; ++a) {
       const F w = F(bins.weight(a, bin));

@@ -365,3 +365,22 @@ def test_height_avg_limits_in_the_rewrite(model, oracle):
                               osc_for(NOM, nt), 0.0, np.ones((2, 3, len(ev), 1, 1)),
                               [0.0, 30.0])
         assert np.abs(z - zr).max() < 1e-10
+
+
+def test_height_avg_wide_bins(model, oracle):
+    """Bins wide enough (down to 20 MeV, 80-300 km) that a pair's phase
+    across a bin exceeds half a revolution, where the sinc divides the
+    reduced sine instead of using the division-free quotient."""
+    e = W.earth_12()
+    cz = np.linspace(-1.0, 1.0, 41)
+    ev = W.log_grid(37, 0.02, 100.0)
+    edges = np.array([0.0, 80.0, 300.0])
+    pr = np.random.default_rng(6).random((2, 3, len(ev), len(cz), 2))
+    pr /= pr.sum(axis=-1, keepdims=True)
+    for nt in (0, 1):
+        ref = oracle.atmos_pha(nt, cz, ev, e.radii, e.rhos, e.yps, osc_for(NOM, nt),
+                               18.0, pr, edges)
+        assert np.abs(m_atmos_pha(model, 0, nt, cz, ev, e, NOM, 18.0, pr, edges)
+                      - ref).max() < 1e-10
+        assert np.abs(m_atmos_pha(model, 2, nt, cz, ev, e, NOM, 18.0, pr, edges)
+                      - ref).max() < 1e-5
