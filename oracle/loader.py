@@ -59,6 +59,13 @@ def _atmos_pha_argtypes(oracle: bool):
             ([C.c_int] if oracle else []) + [C.c_int, _dp, C.c_char_p, C.c_int])
 
 
+def _atmos_poly_pha_argtypes(oracle: bool):
+    return ([C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int,
+             C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+             C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp] +
+            ([C.c_int] if oracle else []) + [C.c_int, _dp, C.c_char_p, C.c_int])
+
+
 def _beam_argtypes(outp):
     return [C.c_int, _dp, C.c_int, C.c_double, C.c_double, C.c_double,
             C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
@@ -107,6 +114,9 @@ class PropLib:
         self._atmos_pha = getattr(L, prefix + "atmos_pha_f64")
         self._atmos_pha.argtypes = _atmos_pha_argtypes(prefix == "orc_")
         self._atmos_pha.restype = C.c_int
+        self._atmos_poly_pha = getattr(L, prefix + "atmos_poly_pha_f64")
+        self._atmos_poly_pha.argtypes = _atmos_poly_pha_argtypes(prefix == "orc_")
+        self._atmos_poly_pha.restype = C.c_int
         self._beam = getattr(L, prefix + "beam_f64")
         self._beam.argtypes = _beam_argtypes(_dp)
         self._beam.restype = C.c_int
@@ -181,6 +191,27 @@ class PropLib:
                              *osc.args(), float(h_km), nb, _ptr(prob),
                              _ptr(edges_km), *extra, int(threads), _ptr(out),
                              err, 512)
+        self._check(rc, err)
+        return out
+
+    def atmos_poly_pha(self, nutype: int, cosv, ev, radii, a, b, c, yps, osc: Osc,
+                       h_km: float, prob, edges_km, threads: int = 1,
+                       int_abs_sinc: bool = False):
+        """Production-height averaging through a polynomial model."""
+        arrs = [np.ascontiguousarray(x, dtype=np.float64)
+                for x in (cosv, ev, radii, a, b, c, yps, prob, edges_km)]
+        cosv, ev, radii, a, b, c, yps, prob, edges_km = arrs
+        nb = len(edges_km) - 1
+        if prob.size != nb * 2 * 3 * len(ev) * len(cosv):
+            raise ValueError("prob must hold 2 * 3 * nE * nC * nb values")
+        err = C.create_string_buffer(512)
+        out = np.zeros((len(cosv), len(ev), 9), dtype=np.float64)
+        extra = [int(bool(int_abs_sinc))] if self.prefix == "orc_" else []
+        rc = self._atmos_poly_pha(int(nutype), _ptr(cosv), len(cosv), _ptr(ev), len(ev),
+                                  _ptr(radii), _ptr(a), _ptr(b), _ptr(c), _ptr(yps),
+                                  len(radii), *osc.args(), float(h_km), nb, _ptr(prob),
+                                  _ptr(edges_km), *extra, int(threads), _ptr(out),
+                                  err, 512)
         self._check(rc, err)
         return out
 

@@ -808,6 +808,34 @@ int orc_atmos_pha_f64(int nutype, const double* cosv, int nC, const double* ev,
   return 0;
 }
 
+// Production-height averaging through a polynomial Earth model.
+int orc_atmos_poly_pha_f64(int nutype, const double* cosv, int nC, const double* ev,
+                           int nE, const double* radii, const double* pa,
+                           const double* pb, const double* pc, const double* yps,
+                           int nrows, double th12, double th13, double th23,
+                           double dcp, int kNuType, double dm12, double dm23,
+                           double h_km, int nb, const double* prob,
+                           const double* edges_km, int int_abs_sinc, int threads,
+                           double* out, char* err, int errlen) {
+  try {
+    if (nb < 0 || nb > 20)
+      throw std::runtime_error("invalid number of production height bins");
+    orc::Osc o;
+    orc::make_osc(o, th12, th13, th23, dcp, kNuType, dm12, dm23);
+    orc::Earth e;
+    auto v = [&](const double* x) { return std::vector<double>(x, x + nrows); };
+    orc::set_density_poly(e, v(radii), v(pa), v(pb), v(pc), v(yps));
+    orc::set_threads(threads);
+    const orc::Pha pha{nb, prob, edges_km, int_abs_sinc != 0};
+    orc::calculate_atmos(o, e, nutype != 0, cosv, nC, ev, nE,
+                         h_km * 100000.0, out, &pha);
+  } catch (const std::exception& ex) {
+    orc::put_err(err, errlen, ex.what());
+    return -1;
+  }
+  return 0;
+}
+
 // The same with a polynomial Earth model (setDensity(radii, a, b, c, yps)).
 int orc_atmos_poly_f64(int nutype, const double* cosv, int nC, const double* ev,
                        int nE, const double* radii, const double* pa,

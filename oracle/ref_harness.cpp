@@ -128,12 +128,13 @@ struct AveragingPropagator : cudaprob3linear::AtmosCpuPropagator<T> {
   void enableAveraging() { this->useProductionHeightAveraging = true; }
 };
 
+// `poly` (a, b, c per row) selects setDensity(radii, a, b, c, yps).
 int run_atmos_pha(int nutype, const double* cosv, int nC, const double* ev, int nE,
                   const double* radii, const double* rhos, const double* yps,
-                  int nrows, double th12, double th13, double th23, double dcp,
-                  int kNuType, double dm12, double dm23, double h_km, int nb,
-                  const double* prob, const double* edges_km, int threads,
-                  double* out, char* err, int errlen) {
+                  int nrows, const double* const* poly, double th12, double th13,
+                  double th23, double dcp, int kNuType, double dm12, double dm23,
+                  double h_km, int nb, const double* prob, const double* edges_km,
+                  int threads, double* out, char* err, int errlen) {
   using namespace cudaprob3linear;
   try {
     AveragingPropagator<double> p(nC, nE, threads);
@@ -141,8 +142,13 @@ int run_atmos_pha(int nutype, const double* cosv, int nC, const double* ev, int 
     p.setCosineList(to_vec<double>(cosv, nC));
     p.setMNSMatrix(th12, th13, th23, dcp, kNuType);
     p.setNeutrinoMasses(dm12, dm23);
-    p.setDensity(to_vec<double>(radii, nrows), to_vec<double>(rhos, nrows),
-                 to_vec<double>(yps, nrows));
+    if (poly)
+      p.setDensity(to_vec<double>(radii, nrows), to_vec<double>(poly[0], nrows),
+                   to_vec<double>(poly[1], nrows), to_vec<double>(poly[2], nrows),
+                   to_vec<double>(yps, nrows));
+    else
+      p.setDensity(to_vec<double>(radii, nrows), to_vec<double>(rhos, nrows),
+                   to_vec<double>(yps, nrows));
     p.setProductionHeight(h_km);
     p.SetNumberOfProductionHeightBinsForAveraging(nb);
     p.enableAveraging();
@@ -249,8 +255,23 @@ int ref_atmos_pha_f64(int nutype, const double* cosv, int nC, const double* ev,
                       double dm23, double h_km, int nb, const double* prob,
                       const double* edges_km, int threads, double* out, char* err,
                       int errlen) {
-  return run_atmos_pha(nutype, cosv, nC, ev, nE, radii, rhos, yps, nrows, th12, th13,
-                       th23, dcp, kNuType, dm12, dm23, h_km, nb, prob, edges_km,
+  return run_atmos_pha(nutype, cosv, nC, ev, nE, radii, rhos, yps, nrows, nullptr, th12,
+                       th13, th23, dcp, kNuType, dm12, dm23, h_km, nb, prob, edges_km,
+                       threads, out, err, errlen);
+}
+
+// The same through a polynomial Earth model.
+int ref_atmos_poly_pha_f64(int nutype, const double* cosv, int nC, const double* ev,
+                           int nE, const double* radii, const double* pa,
+                           const double* pb, const double* pc, const double* yps,
+                           int nrows, double th12, double th13, double th23,
+                           double dcp, int kNuType, double dm12, double dm23,
+                           double h_km, int nb, const double* prob,
+                           const double* edges_km, int threads, double* out,
+                           char* err, int errlen) {
+  const double* const poly[3] = {pa, pb, pc};
+  return run_atmos_pha(nutype, cosv, nC, ev, nE, radii, nullptr, yps, nrows, poly, th12,
+                       th13, th23, dcp, kNuType, dm12, dm23, h_km, nb, prob, edges_km,
                        threads, out, err, errlen);
 }
 

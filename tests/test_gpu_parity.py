@@ -430,3 +430,33 @@ def test_height_avg_errors():
         p.SetNumberOfProductionHeightBinsForAveraging(2)  # turns it off again
         with pytest.raises(Cp3Error, match="not expecting"):
             p.setProductionHeightList(np.ones(144), [1.0, 2.0, 3.0])
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("mixed", TOL_MIXED)])
+def test_height_avg_poly_vs_oracle(oracle, precision, tol):
+    """Averaging through a polynomial model (per-trajectory solves, one
+    cosine per wave), against the oracle's averaging through the same model,
+    which equals the compiled reference's bitwise with its sinc."""
+    from cudaprob3_amd import AtmosPropagator
+    ep = W.prem4_poly()
+    cz = np.concatenate([W.cos_grid(50), np.linspace(0.0, 0.95, 10)])
+    ev = W.log_grid(90, 0.1, 100.0)
+    edges = np.array([3.0, 11.0, 22.0, 35.0, 47.0])
+    pr = height_prob(len(cz), len(ev), len(edges) - 1, 12)
+    for nt in (0, 1):
+        ref = oracle.atmos_poly_pha(nt, cz, ev, ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                                    osc_for(NOM, nt), 22.0, pr, edges, threads=4)
+        with AtmosPropagator(len(cz), len(ev), precision=precision) as p:
+            p.setEnergyList(ev)
+            p.setCosineList(cz)
+            p.setMNSMatrix(W.TH12, W.TH13, W.TH23, W.DCP, -1 if nt else 1)
+            p.setNeutrinoMasses(W.DM12, W.DM23)
+            p.setDensity(ep.radii, ep.a, ep.b, ep.c, ep.yps)
+            p.setProductionHeight(22.0)
+            p.SetNumberOfProductionHeightBinsForAveraging(len(edges) - 1)
+            p.setProductionHeightAveraging(True)
+            p.setProductionHeightList(pr.ravel(), edges)
+            p.calculateProbabilities(nt)
+            got = p.probabilities()
+        print(precision, nt, "max|d|", np.abs(got - ref).max())
+        assert np.abs(got - ref).max() < tol

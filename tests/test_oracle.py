@@ -506,3 +506,25 @@ def test_height_avg_weights(oracle):
     assert np.array_equal(run(q), a)
     assert not np.array_equal(run(q, 1), run(p1, 1))
     del osc
+
+
+def test_height_avg_poly_reference(oracle):
+    """Averaging through a polynomial model, bitwise against the compiled
+    reference where it is built (grazing cosines, where both return NaN,
+    left out)."""
+    from oracle.loader import load_ref
+    ref = load_ref()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs the reference sources)")
+    ep = W.prem4_poly()
+    cz = np.concatenate([W.cos_grid(12), np.linspace(0.0, 0.9, 4)])
+    ev = W.log_grid(10, 0.1, 50.0)
+    edges = np.array([3.0, 11.0, 22.0, 47.0])
+    prob = _bin_prob(len(cz), len(ev), 3, 71)
+    for nt in (0, 1):
+        a = ref.atmos_poly_pha(nt, cz, ev, ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                               osc_for(NOM, nt), 22.0, prob, edges)
+        b = oracle.atmos_poly_pha(nt, cz, ev, ep.radii, ep.a, ep.b, ep.c, ep.yps,
+                                  osc_for(NOM, nt), 22.0, prob, edges,
+                                  int_abs_sinc=True)
+        assert np.array_equal(a, b)
