@@ -86,3 +86,37 @@ def test_beam_batch_4096x1e5_one_launch(oracle):
           f"{rate:.3e} evals/s; sampled 16 sets: max {worst:.2e} "
           f"rms {np.sqrt(sq / n):.2e}")
     assert worst < 1e-5
+
+
+def test_single_grid_output_past_2_31_elements(oracle):
+    """One grid of 16384 x 16384 points in mixed precision: its [9][nC][nE]
+    output holds 2^31 + 2^28 elements (9.7 GB), the last channel starting at
+    element 2^31.  Sampled cosines (first, middle, last) and energies in every
+    channel against the oracle, so 64-bit indexing is exercised on the device
+    and in the host accessors."""
+    from cudaprob3_amd import AtmosPropagator
+    n = 16384
+    e = W.prem4()
+    cz = -1.0 + 2.0 * np.arange(n) / n
+    ev = W.log_grid(n, 0.1, 100.0)
+    rows = [0, 8191, n - 1]
+    sel = list(range(0, n, 257)) + [n - 1]
+    t0 = time.perf_counter()
+    with AtmosPropagator(n, n, precision="mixed") as p:
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        p.setMNSMatrix(W.TH12, W.TH13, W.TH23, W.DCP, 1)
+        p.setNeutrinoMasses(W.DM12, W.DM23)
+        p.setDensity(e.radii, e.rhos, e.yps)
+        p.setProductionHeight(22.0)
+        p.calculateProbabilities(0)
+        p.calculateProbabilities(0)  # the second launch, timed without first-use costs
+        ms = p.lastKernelMs()
+        got = np.array([[[p.getProbability(ic, ie, t) for t in range(9)] for ie in sel]
+                        for ic in rows])
+    ref = oracle.atmos(0, cz[rows], ev[sel], e.radii, e.rhos, e.yps,
+                       osc_for(NOM, 0), 22.0, threads=4)
+    err = np.abs(got - ref).max()
+    print(f"16384 x 16384 mixed: kernel {ms:.2f} ms ({n * n / ms * 1e3:.3g} evals/s), "
+          f"max|d| {err:.2e} on {got.size} samples, {time.perf_counter() - t0:.1f} s")
+    assert err < 1e-5
