@@ -70,7 +70,8 @@ struct cp3_batch {
 
   int tune_nc = 0, tune_w = 0, tune_waves = 0;  // 0 = automatic
   int tune_mode = CP3_SOLVE_AUTO;
-  int nc = 1, w = 1, mode = CP3_SOLVE_SHARED, a_grid = 0, b_grid = 0;
+  int nc = 1, w = 1, mode = CP3_SOLVE_SHARED, a_grid = 0;
+  int64_t b_max_units = 0;  // units of the longest beam job
 
   bool use_graph = false;
   hipGraph_t graph = nullptr;
@@ -306,15 +307,17 @@ struct cp3_batch {
     }
     t_units = u;
     u = 0;
+    b_max_units = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
       if (jobs[j].kind != CP3_JOB_BEAM) continue;
       cp3::BeamJob& b = bjobs[(size_t)slot[j]];
       b.unit_begin = u;
-      u += cp3::beam_units(bgrids[(size_t)jobs[j].grid]->nE);
+      const int64_t n = cp3::beam_units(bgrids[(size_t)jobs[j].grid]->nE);
+      u += n;
+      b_max_units = std::max(b_max_units, n);
     }
     b_units = u;
     a_grid = grid_for(a_units, tune_waves, true);
-    b_grid = grid_for(b_units, tune_waves, false);
     if (mode == CP3_SOLVE_TABLE && a_grid > 0) {
       // a multiple of 8 (one share per XCD), at most one block per unit
       const int64_t cap = (a_units + 7) / 8 * 8;
@@ -377,7 +380,7 @@ struct cp3_batch {
     mark(3);
     if (b_units > 0)
       hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)d_bjobs.p,
-                                       (int)bjobs.size(), b_units, b_grid, stream),
+                                       (int)bjobs.size(), b_max_units, stream),
                 "batched beam launch");
     mark(4);
   }

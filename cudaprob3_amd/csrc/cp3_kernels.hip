@@ -597,11 +597,11 @@ __global__ void __launch_bounds__(kWave) atmos_table_avg_kernel(
 }
 
 template <class P>
-__device__ __forceinline__ void beam_unit(const BeamJob& job, int64_t local) {
+__device__ __forceinline__ void beam_unit(const BeamJob& job, int64_t local, int lane) {
   using D = typename P::D;
   using F = typename P::F;
   using O = typename OutT<P>::type;
-  const int e = (int)local * kWave + (int)threadIdx.x;
+  const int e = (int)local * kWave + lane;
   const bool e_ok = e < job.nE;
   const D E = (D)job.ev[e_ok ? e : job.nE - 1];
   F prob[9];
@@ -615,26 +615,43 @@ __device__ __forceinline__ void beam_unit(const BeamJob& job, int64_t local) {
 
 template <class P>
 __global__ void __launch_bounds__(kWave) beam_single_kernel(BeamJob job) {
-  beam_unit<P>(job, (int64_t)blockIdx.x);
+  beam_unit<P>(job, (int64_t)blockIdx.x, (int)threadIdx.x);
 }
 
-// One work unit (64 energies of one job) per workgroup: without a loop the
-// solve's constants are not held in registers across units, which keeps the
-// kernel at about 50 VGPRs (8 waves per SIMD instead of 4).
+// 128 energies of one job per wave, two per lane (energies e and e + 64: two
+// independent solves in flight), kBeamWaves waves per workgroup.  Without a
+// loop the solve's constants are not held in registers across units, which
+// keeps the kernel at 46-73 VGPRs (6-8 waves per SIMD).  The grid is 2-D (y:
+// job, x: groups of kBeamWaves 128-energy units), so a wave finds its job
+// without a search; waves past a shorter job's energies exit at once.
+constexpr int kBeamWaves = 4;
 template <class P>
-__global__ void __launch_bounds__(kWave) beam_batch_kernel(
-    const BeamJob* __restrict__ jobs, int njobs, int64_t unit0) {
-  const int64_t u = unit0 + blockIdx.x;
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].unit_begin <= u)
-      lo = mid;
-    else
-      hi = mid - 1;
+__global__ void __launch_bounds__(kWave * kBeamWaves) beam_batch_kernel(
+    const BeamJob* __restrict__ jobs, int job0) {
+  using D = typename P::D;
+  using F = typename P::F;
+  using O = typename OutT<P>::type;
+  const BeamJob& job = jobs[job0 + (int)blockIdx.y];
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+  const int64_t local = (int64_t)blockIdx.x * kBeamWaves + wave;
+  if (local * 2 * kWave >= job.nE) return;
+  const int lane = (int)threadIdx.x % kWave;
+  const int e0 = (int)local * 2 * kWave + lane, e1 = e0 + kWave;
+  const bool ok0 = e0 < job.nE, ok1 = e1 < job.nE;
+  const D E0 = (D)job.ev[ok0 ? e0 : job.nE - 1];
+  const D E1 = (D)job.ev[ok1 ? e1 : job.nE - 1];
+  F p0[9], p1[9];
+  beam_point<P>(job.sc, E0, (D)job.rye, (D)job.L, p0);
+  beam_point<P>(job.sc, E1, (D)job.rye, (D)job.L, p1);
+  O* out = (O*)job.out;
+  if (ok0) {
+#pragma unroll
+    for (int ch = 0; ch < 9; ++ch) out[(int64_t)ch * job.nE + e0] = (O)p0[ch];
   }
-  const BeamJob& job = jobs[lo];
-  beam_unit<P>(job, u - job.unit_begin);
+  if (ok1) {
+#pragma unroll
+    for (int ch = 0; ch < 9; ++ch) out[(int64_t)ch * job.nE + e1] = (O)p1[ch];
+  }
 }
 
 // ------------------------------------------------------------- launchers
@@ -883,19 +900,19 @@ hipError_t launch_beam_single(int precision, const BeamJob& job,
 }
 
 hipError_t launch_beam_batch(int precision, const BeamJob* jobs, int njobs,
-                             int64_t nunits, int grid, hipStream_t st) {
-  if (nunits <= 0 || njobs <= 0) return hipSuccess;
-  if (grid <= 0) return hipErrorInvalidConfiguration;
-  // one workgroup per unit, in launches of at most 2^31 - 1 workgroups (the
-  // grid argument, a persistent grid size, is not used by this kernel)
-  (void)grid;
-  const int64_t kMaxGrid = 0x7fffffff;
-  for (int64_t u0 = 0; u0 < nunits; u0 += kMaxGrid) {
-    const dim3 g((unsigned)std::min<int64_t>(nunits - u0, kMaxGrid)), b(kWave);
+                             int64_t max_job_units, hipStream_t st) {
+  if (max_job_units <= 0 || njobs <= 0) return hipSuccess;
+  if (max_job_units > 0x7fffffff) return hipErrorInvalidConfiguration;
+  // at most 65535 jobs per launch (grid y)
+  for (int j0 = 0; j0 < njobs; j0 += 65535) {
+    const int64_t pairs = (max_job_units + 1) / 2;  // 128-energy units
+    const dim3 g((unsigned)((pairs + kBeamWaves - 1) / kBeamWaves),
+                 (unsigned)std::min(njobs - j0, 65535)),
+        b(kWave * kBeamWaves);
     switch (precision) {
-      case kFP64: beam_batch_kernel<PolicyF64><<<g, b, 0, st>>>(jobs, njobs, u0); break;
-      case kFP32: beam_batch_kernel<PolicyF32><<<g, b, 0, st>>>(jobs, njobs, u0); break;
-      case kMixed: beam_batch_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs, njobs, u0); break;
+      case kFP64: beam_batch_kernel<PolicyF64><<<g, b, 0, st>>>(jobs, j0); break;
+      case kFP32: beam_batch_kernel<PolicyF32><<<g, b, 0, st>>>(jobs, j0); break;
+      case kMixed: beam_batch_kernel<PolicyMixed><<<g, b, 0, st>>>(jobs, j0); break;
       default: return hipErrorInvalidValue;
     }
     const hipError_t e = hipGetLastError();
@@ -926,9 +943,9 @@ hipError_t atmos_batch_occupancy(int precision, int nc, int w, int* n) {
 
 hipError_t beam_batch_occupancy(int precision, int* n) {
   switch (precision) {
-    case kFP64: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyF64>, kWave, 0);
-    case kFP32: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyF32>, kWave, 0);
-    case kMixed: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyMixed>, kWave, 0);
+    case kFP64: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyF64>, kWave * kBeamWaves, 0);
+    case kFP32: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyF32>, kWave * kBeamWaves, 0);
+    case kMixed: return hipOccupancyMaxActiveBlocksPerMultiprocessor(n, beam_batch_kernel<PolicyMixed>, kWave * kBeamWaves, 0);
     default: return hipErrorInvalidValue;
   }
 }

@@ -151,10 +151,11 @@ hipError_t launch_atmos_tail(int precision, int nc, int w, const AtmosJob* d_job
                              hipStream_t stream);
 hipError_t launch_beam_single(int precision, const BeamJob& job,
                               hipStream_t stream);
-// The batched beam kernel runs one workgroup per unit whatever `grid` asks
-// (a persistent loop would hold the solve's constants in VGPRs).
+// The batched beam kernel on a (128-energy groups of the longest job) x (job)
+// grid; max_job_units counts the longest job's 64-energy units (beam_units).
+// No persistent loop: it would hold the solve's constants in VGPRs.
 hipError_t launch_beam_batch(int precision, const BeamJob* d_jobs, int njobs,
-                             int64_t nunits, int grid, hipStream_t stream);
+                             int64_t max_job_units, hipStream_t stream);
 
 // Resident 64-lane workgroups per CU for the batched kernels (occupancy).
 hipError_t atmos_batch_occupancy(int precision, int nc, int w, int* blocks_per_cu);

@@ -215,7 +215,7 @@ struct cp3_server {
     peak_mem = std::max(peak_mem, alloc->used());
     s.ajobs.clear();
     s.bjobs.clear();
-    int64_t au = 0, bu = 0, max_waves = 0;
+    int64_t au = 0, bu = 0, bmax = 0, max_waves = 0;
     for (int64_t id : take) {
       const Request& r = reqs[id];
       const cp3_job& in = r.job;
@@ -249,6 +249,7 @@ struct cp3_server {
         b.out = out;
         b.unit_begin = bu;
         bu += cp3::beam_units(g.nE);
+        bmax = std::max<int64_t>(bmax, cp3::beam_units(g.nE));
         s.bjobs.push_back(b);
       }
     }
@@ -282,8 +283,7 @@ struct cp3_server {
       }
       if (bu > 0)
         hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)s.d_bjobs.p,
-                                         (int)s.bjobs.size(), bu,
-                                         (int)std::min<int64_t>(bu, 0x7fffffff), s.stream),
+                                         (int)s.bjobs.size(), bmax, s.stream),
                   "beam launch");
       hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
     } catch (const std::exception& e) {
