@@ -77,10 +77,16 @@ void auto_tuning(int precision, const cp3::GridDev& g, int* nc, int* w) {
   if (cp3::atmos_units(g, 1, *w) * *w < 4096) *w = 1;
 }
 
-// Slab-table mode: the largest cosine block up to 4 that still leaves 2048
-// work units (8 per CU), else 1.
-int table_nc(const cp3::GridDev& g) {
-  for (int c = 4; c > 1; c /= 2)
+// Slab-table mode: the largest cosine block up to a cap that still leaves
+// 2048 work units (8 per CU), else 1.  The cap is 4, or on models of 32 or
+// more shells 2 in mixed and 1 in FP32 precision (2000 x 2000 grids on
+// MI355X: 12 shells, mixed block 4 / 2 / 1 = 0.077 / 0.080 / 0.083 ms;
+// 59 shells, mixed 0.172 / 0.153 / 0.174 ms, FP32 0.168 / 0.151 / 0.147 ms;
+// FP64 blocks 4 and 1 within 1% of each other on both).
+int table_nc(int precision, const cp3::GridDev& g) {
+  int cap = 4;
+  if (g.S >= 32) cap = precision == CP3_MIXED ? 2 : (precision == CP3_FP32 ? 1 : 4);
+  for (int c = cap; c > 1; c /= 2)
     if (cp3::cosine_blocks(g, c, 1) * cp3::energy_tiles(g) >= 2048) return c;
   return 1;
 }
@@ -226,7 +232,7 @@ struct cp3_atmos {
                 "atmospheric kernel launch");
     } else {
       // the grid's slab table (a pre-pass) read by the propagation kernel
-      const int nc = nc_force ? nc_force : table_nc(job.g);
+      const int nc = nc_force ? nc_force : table_nc(precision, job.g);
       d_slabs.alloc((size_t)cp3::slab_entries(job.g) * cp3::slab_entry_bytes(precision), dev);
       job.slabs = d_slabs.p;
       d_job.alloc(sizeof(job), dev);
