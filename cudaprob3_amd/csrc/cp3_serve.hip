@@ -127,7 +127,9 @@ double us_between(Clock::time_point a, Clock::time_point b) {
 
 struct cp3_server {
   int dev = 0, precision = CP3_MIXED;
-  int nc = 2, w = 4;
+  // cosine block (0: automatic, 1 in table mode and 2 in shared mode: a
+  // request's grid then spreads over the most work units) and group waves
+  int nc = 0, w = 4;
   // atmospheric batches pre-solve their slabs into a table (CP3_SOLVE_TABLE,
   // the default) or solve inside the propagation kernel (CP3_SOLVE_SHARED)
   int solve_mode = CP3_SOLVE_TABLE;
@@ -200,6 +202,7 @@ struct cp3_server {
   bool launch_batch(Slot& s) {
     std::vector<int64_t> take;
     const bool table = solve_mode == CP3_SOLVE_TABLE;
+    const int nc = this->nc ? this->nc : (table ? 1 : 2);
     while (!pending.empty() && (int)take.size() < max_batch) {
       Request& r = reqs[pending.front()];
       const size_t tb = table ? table_bytes(r.job) : 0;

@@ -311,7 +311,8 @@ int cp3_server_create(int device, int precision, size_t mem_cap_bytes,
                       int n_slots, int max_batch_jobs, cp3_server** out);
 /* Drains admitted work, fails the rest, frees everything. */
 int cp3_server_destroy(cp3_server* s);
-/* Kernel tuning for every batch (cosine block, group waves). */
+/* Kernel tuning for every batch (cosine block, group waves).  By default the
+ * cosine block is 1 in table mode and 2 in shared mode, with 4 group waves. */
 int cp3_server_set_tuning(cp3_server* s, int cosine_block, int group_waves);
 /* Solve mode of the atmospheric batches: CP3_SOLVE_TABLE (and AUTO, the
  * default) pre-solves each request's slabs into a table that lives in the
