@@ -395,7 +395,12 @@ def main():
         result["p99_us"] = result["serving"]["mixed"]["p99_us"]
 
     if not args.no_configs:
-        result["configs"] = other_configs()
+        # a failure here must not cost the headline line
+        try:
+            result["configs"] = other_configs()
+        except Exception as ex:  # noqa: BLE001
+            print(f"configs failed: {ex!r}", file=sys.stderr)
+            result["configs"] = {"error": repr(ex)}
 
     if not args.no_cpu:
         cb, worst, rms = cpu_baseline(wl, mixed_results)
