@@ -280,3 +280,22 @@ def test_random_grid_shapes(precision, mode):
         ref = single_atmos(job, earth, precision)
         assert np.array_equal(b.result(i), ref), (i, job.grid, job.nutype)
     b.close()
+
+
+def test_beam_batch_over_65535_jobs():
+    """70000 beam jobs (100 energies each, two grids of different lengths):
+    the beam launch runs in slices of 65535 jobs; sampled jobs from every
+    slice, on both grids, equal the single propagator bitwise."""
+    sets = W.draw_param_sets(7, 70000)
+    ev_a, ev_b = W.log_grid(100, 0.1, 20.0), W.log_grid(37, 0.3, 9.0)
+    with Batch("mixed") as b:
+        ga = b.add_beam_grid(ev_a, 2.6, 1300.0)
+        gb = b.add_beam_grid(ev_b, 2.6, 1300.0)
+        n = 70000
+        jobs = [make_job(BEAM, ga if j % 3 else gb, sets[j % 7], j % 2) for j in range(n)]
+        b.set_jobs(jobs)
+        b.run()
+        b.wait()
+        for j in (0, 1, 2, 65534, 65535, 65536, 69998, 69999):
+            ev = ev_a if j % 3 else ev_b
+            assert np.array_equal(b.result(j), single_beam(sets[j % 7], j % 2, ev, "mixed")), j
