@@ -65,7 +65,10 @@ int cp3_device_count(int* n);
 
 /* AtmosCudaPropagatorSingle(id, n_cosines, n_energies)
  * (atmoscudapropagator.cu:30-62).  precision: CP3_FP64 / CP3_FP32 /
- * CP3_MIXED.  Allocates the device result buffer and a stream. */
+ * CP3_MIXED.  Allocates the device result buffer and a stream.  An empty grid
+ * (n_cosines or n_energies <= 0) is CP3_ERR_INVALID here; the reference
+ * accepts it and then launches an empty grid, a CUDA error it ends the
+ * process on. */
 int cp3_atmos_create(int device, int n_cosines, int n_energies, int precision,
                      cp3_atmos** out);
 /* ~AtmosCudaPropagatorSingle (atmoscudapropagator.cu:71-76). */

@@ -460,3 +460,16 @@ def test_height_avg_poly_vs_oracle(oracle, precision, tol):
             got = p.probabilities()
         print(precision, nt, "max|d|", np.abs(got - ref).max())
         assert np.abs(got - ref).max() < tol
+
+
+def test_empty_grids_rejected():
+    """An empty grid is refused at construction with a status code (the
+    reference accepts it and ends the process on the empty kernel launch,
+    atmoscudapropagator.cu:284-287 with hpc_helpers.cuh:71-78)."""
+    from cudaprob3_amd import AtmosPropagator, BeamPropagator
+    from cudaprob3_amd._native import Cp3Error
+    for nc, ne in ((0, 10), (10, 0), (0, 0)):
+        with pytest.raises(Cp3Error, match="must be positive"):
+            AtmosPropagator(nc, ne)
+    with pytest.raises(Cp3Error, match="must be positive"):
+        BeamPropagator(0)
