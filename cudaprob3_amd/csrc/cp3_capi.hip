@@ -180,6 +180,14 @@ struct cp3_atmos {
     if (!height_set)
       throw Error(CP3_ERR_STATE,
                   "CudaPropagatorSingle::calculateProbabilities. production height was not set");
+    // Without a model the trajectory table has no shells and no density row,
+    // which every kernel reads (the reference CPU class would propagate
+    // through the atmosphere segment alone, its GPU class through empty
+    // device arrays).
+    if (!earth.set)
+      throw Error(CP3_ERR_STATE,
+                  "CudaPropagatorSingle::calculateProbabilities. density model was not set "
+                  "(setDensity or setDensityFromFile)");
     if (avg && !avg_list)
       throw Error(CP3_ERR_STATE,
                   "CpuPropagator::calculateProbabilities. production height array was not set, "

@@ -105,6 +105,9 @@ struct Request {
   Clock::time_point t_submit, t_launch, t_done;
   ReqState state = kPending;
   bool has_mem = false;
+  // set while cp3_server_fetch copies the result out of the arena: release
+  // and a second fetch are refused until the copy is done
+  bool fetching = false;
   std::string err;
 };
 
@@ -113,6 +116,11 @@ struct Slot {
   hipEvent_t done = nullptr;
   bool busy = false;
   std::vector<int64_t> reqs;
+  // the admitted batch's launch shape (set under the server lock by admit,
+  // read by enqueue without it)
+  bool table = false;
+  int nc = 1, w = 1, precision = CP3_MIXED;
+  int64_t au = 0, bu = 0, bmax = 0, max_waves = 0;
   std::vector<cp3::AtmosJob> ajobs;
   std::vector<cp3::BeamJob> bjobs;
   DevBuf d_ajobs, d_bjobs;
@@ -197,9 +205,11 @@ struct cp3_server {
                                    : bgrids[(size_t)j.grid]->points();
   }
 
-  // Called with mu held.  Returns false when the head of the queue does not
-  // fit in the arena (it waits for memory; admission stays FIFO).
-  bool launch_batch(Slot& s) {
+  // Admission, called with mu held: takes the requests at the head of the
+  // queue whose output (and slab table) fit in the arena, in arrival order,
+  // and builds their job tables in the slot.  Returns false when the head of
+  // the queue does not fit (it waits for memory; admission stays FIFO).
+  bool admit(Slot& s) {
     std::vector<int64_t> take;
     const bool table = solve_mode == CP3_SOLVE_TABLE;
     const int nc = this->nc ? this->nc : (table ? 1 : 2);
@@ -218,7 +228,11 @@ struct cp3_server {
     peak_mem = std::max(peak_mem, alloc->used());
     s.ajobs.clear();
     s.bjobs.clear();
-    int64_t au = 0, bu = 0, bmax = 0, max_waves = 0;
+    s.au = s.bu = s.bmax = s.max_waves = 0;
+    s.table = table;
+    s.nc = nc;
+    s.w = w;
+    s.precision = precision;
     for (int64_t id : take) {
       const Request& r = reqs[id];
       const cp3_job& in = r.job;
@@ -231,14 +245,14 @@ struct cp3_server {
         a.sc = sc;
         a.g = agrids[(size_t)in.grid]->dev();
         a.out = out;
-        a.unit_begin = au;
+        a.unit_begin = s.au;
         a.etiles = cp3::energy_tiles(a.g);
         if (table) {
           a.slabs = (char*)out + ArenaAlloc::round(r.bytes);
-          au += cp3::atmos_units(a.g, nc, 1);
-          max_waves = std::max(max_waves, cp3::slab_waves(a.g));
+          s.au += cp3::atmos_units(a.g, nc, 1);
+          s.max_waves = std::max(s.max_waves, cp3::slab_waves(a.g));
         } else {
-          au += cp3::atmos_units(a.g, nc, w);
+          s.au += cp3::atmos_units(a.g, nc, w);
         }
         s.ajobs.push_back(a);
       } else {
@@ -250,65 +264,77 @@ struct cp3_server {
         b.rye = g.rye;
         b.L = g.L;
         b.out = out;
-        b.unit_begin = bu;
-        bu += cp3::beam_units(g.nE);
-        bmax = std::max<int64_t>(bmax, cp3::beam_units(g.nE));
+        b.unit_begin = s.bu;
+        s.bu += cp3::beam_units(g.nE);
+        s.bmax = std::max<int64_t>(s.bmax, cp3::beam_units(g.nE));
         s.bjobs.push_back(b);
       }
     }
-    try {
-      const size_t abytes = s.ajobs.size() * sizeof(cp3::AtmosJob);
-      const size_t bbytes = s.bjobs.size() * sizeof(cp3::BeamJob);
-      std::memcpy(s.staging.p, s.ajobs.data(), abytes);
-      std::memcpy((char*)s.staging.p + abytes, s.bjobs.data(), bbytes);
-      if (abytes)
-        hip_check(hipMemcpyAsync(s.d_ajobs.p, s.staging.p, abytes, hipMemcpyHostToDevice,
-                                 s.stream),
-                  "job table upload");
-      if (bbytes)
-        hip_check(hipMemcpyAsync(s.d_bjobs.p, (char*)s.staging.p + abytes, bbytes,
-                                 hipMemcpyHostToDevice, s.stream),
-                  "job table upload");
-      const cp3::AtmosJob* aj = (const cp3::AtmosJob*)s.d_ajobs.p;
-      const int na = (int)s.ajobs.size();
-      if (au > 0 && table) {
-        hip_check(cp3::launch_slab_table(precision, aj, na, max_waves, s.stream),
-                  "slab table launch");
-        // one workgroup per unit, a multiple of 8 (one share per XCD)
-        hip_check(cp3::launch_atmos_table(precision, nc, aj, na, au,
-                                          (int)std::min<int64_t>((au + 7) / 8 * 8, 0x7ffffff8),
-                                          s.stream),
-                  "atmospheric launch");
-      } else if (au > 0) {
-        hip_check(cp3::launch_atmos_batch(precision, nc, w, aj, na, au,
-                                          (int)std::min<int64_t>(au, 0x7fffffff), s.stream),
-                  "atmospheric launch");
-      }
-      if (bu > 0)
-        hip_check(cp3::launch_beam_batch(precision, (const cp3::BeamJob*)s.d_bjobs.p,
-                                         (int)s.bjobs.size(), bmax, s.stream),
-                  "beam launch");
-      hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
-    } catch (const std::exception& e) {
-      // work enqueued before the failure may still use the requests' memory
-      (void)hipStreamSynchronize(s.stream);
-      for (int64_t id : take) fail(id, e.what());
+    s.reqs = std::move(take);
+    s.busy = true;
+    return true;
+  }
+
+  // The HIP work of an admitted batch, called without mu: job tables to the
+  // device, the kernels and the completion event, all on the slot's stream.
+  // Only the scheduler thread touches a slot, and the requests of an admitted
+  // batch are neither pending nor finished, so no client call reads or frees
+  // them meanwhile.
+  void enqueue(Slot& s) {
+    const size_t abytes = s.ajobs.size() * sizeof(cp3::AtmosJob);
+    const size_t bbytes = s.bjobs.size() * sizeof(cp3::BeamJob);
+    std::memcpy(s.staging.p, s.ajobs.data(), abytes);
+    std::memcpy((char*)s.staging.p + abytes, s.bjobs.data(), bbytes);
+    if (abytes)
+      hip_check(hipMemcpyAsync(s.d_ajobs.p, s.staging.p, abytes, hipMemcpyHostToDevice,
+                               s.stream),
+                "job table upload");
+    if (bbytes)
+      hip_check(hipMemcpyAsync(s.d_bjobs.p, (char*)s.staging.p + abytes, bbytes,
+                               hipMemcpyHostToDevice, s.stream),
+                "job table upload");
+    const cp3::AtmosJob* aj = (const cp3::AtmosJob*)s.d_ajobs.p;
+    const int na = (int)s.ajobs.size();
+    if (s.au > 0 && s.table) {
+      hip_check(cp3::launch_slab_table(s.precision, aj, na, s.max_waves, s.stream),
+                "slab table launch");
+      // one workgroup per unit, a multiple of 8 (one share per XCD)
+      hip_check(cp3::launch_atmos_table(s.precision, s.nc, aj, na, s.au,
+                                        (int)std::min<int64_t>((s.au + 7) / 8 * 8, 0x7ffffff8),
+                                        s.stream),
+                "atmospheric launch");
+    } else if (s.au > 0) {
+      hip_check(cp3::launch_atmos_batch(s.precision, s.nc, s.w, aj, na, s.au,
+                                        (int)std::min<int64_t>(s.au, 0x7fffffff), s.stream),
+                "atmospheric launch");
+    }
+    if (s.bu > 0)
+      hip_check(cp3::launch_beam_batch(s.precision, (const cp3::BeamJob*)s.d_bjobs.p,
+                                       (int)s.bjobs.size(), s.bmax, s.stream),
+                "beam launch");
+    hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
+  }
+
+  // Called with mu held after enqueue: the batch's requests run (or, if the
+  // enqueue failed, fail and give their memory back).
+  void started(Slot& s, const std::string& error) {
+    if (!error.empty()) {
+      for (int64_t id : s.reqs) fail(id, error);
+      s.reqs.clear();
+      s.busy = false;
       cv_done.notify_all();
-      return true;
+      return;
     }
     const Clock::time_point now = Clock::now();
-    for (int64_t id : take) {
+    for (int64_t id : s.reqs) {
       Request& r = reqs[id];
       r.state = kRunning;
       r.t_launch = now;
     }
-    s.reqs = std::move(take);
-    s.busy = true;
     ++n_batches;
     int64_t inflight = 0;
     for (const auto& o : slots) inflight += o->busy ? 1 : 0;
     peak_inflight = std::max(peak_inflight, inflight);
-    return true;
   }
 
   void fail(int64_t id, const std::string& why) {
@@ -364,10 +390,17 @@ struct cp3_server {
     // stretch each sleep to about 60 us, and a request's completion (or a new
     // submission) is noticed only at the next poll.
     (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0UL, 0UL, 0UL);
+    // Slot state (busy, done, the job tables) is touched by this thread
+    // only; the lock guards the queue, the requests and the arena, and is
+    // released for every HIP call, so clients submit, wait and fetch while
+    // batches are queried and launched.
+    std::vector<std::pair<Slot*, hipError_t>> finished;
     std::unique_lock<std::mutex> lk(mu);
     Clock::time_point last = Clock::now();
     for (;;) {
       bool progress = false, busy = false;
+      finished.clear();
+      lk.unlock();
       for (auto& sp : slots) {
         Slot& s = *sp;
         if (!s.busy) continue;
@@ -376,16 +409,32 @@ struct cp3_server {
           busy = true;
           continue;
         }
-        retire(s, q);
-        progress = true;
+        finished.emplace_back(&s, q);
       }
-      if (progress) cv_done.notify_all();
+      lk.lock();
+      for (auto& f : finished) retire(*f.first, f.second);
+      if (!finished.empty()) {
+        progress = true;
+        cv_done.notify_all();
+      }
       for (auto& sp : slots) {
         Slot& s = *sp;
         if (pending.empty()) break;
         if (s.busy) continue;
-        if (!launch_batch(s)) break;
-        busy = true;
+        if (!admit(s)) break;
+        lk.unlock();
+        std::string error;
+        try {
+          enqueue(s);
+        } catch (const std::exception& e) {
+          // work enqueued before the failure may still use the requests'
+          // memory
+          (void)hipStreamSynchronize(s.stream);
+          error = e.what();
+        }
+        lk.lock();
+        started(s, error);
+        busy = busy || s.busy;
         progress = true;
       }
       if (stop && !busy) break;
@@ -566,10 +615,14 @@ int cp3_server_fetch(cp3_server* s, int64_t id, double* dst) {
       auto it = s->reqs.find(id);
       if (it == s->reqs.end()) throw Error(CP3_ERR_RANGE, "unknown request " + std::to_string(id));
       if (it->second.state != kDone) throw Error(CP3_ERR_STATE, "request is not finished");
+      if (it->second.fetching)
+        throw Error(CP3_ERR_STATE, "request " + std::to_string(id) + " is being fetched");
+      // pinned: its memory stays allocated until the copy below is done
+      it->second.fetching = true;
       off = it->second.off;
       bytes = it->second.bytes;
     }
-    {
+    try {
       DeviceScope ds(s->dev);
       if (s->precision == CP3_FP64) {
         hip_check(hipMemcpy(dst, (const char*)s->arena.p + off, bytes, hipMemcpyDeviceToHost),
@@ -581,6 +634,11 @@ int cp3_server_fetch(cp3_server* s, int64_t id, double* dst) {
                   "result copy");
         for (size_t i = 0; i < tmp.size(); ++i) dst[i] = (double)tmp[i];
       }
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->reqs.find(id);
+      if (it != s->reqs.end()) it->second.fetching = false;
+      throw;
     }
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(id);
@@ -613,6 +671,8 @@ int cp3_server_release(cp3_server* s, int64_t id) {
     if (it == s->reqs.end()) throw Error(CP3_ERR_RANGE, "unknown request " + std::to_string(id));
     if (it->second.state == kPending || it->second.state == kRunning)
       throw Error(CP3_ERR_STATE, "request is not finished");
+    if (it->second.fetching)
+      throw Error(CP3_ERR_STATE, "request " + std::to_string(id) + " is being fetched");
     if (it->second.has_mem) s->alloc->release(it->second.off, it->second.bytes);
     s->reqs.erase(it);
     s->cv_work.notify_one();

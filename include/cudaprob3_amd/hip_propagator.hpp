@@ -1,12 +1,26 @@
 // C++ drop-in propagators for MI355X over the C ABI of libcp3hip.so
 // (include/cp3_hip.h).
 //
-//   HipPropagatorSingle<FLOAT_T>      replaces AtmosCudaPropagatorSingle
-//                                     (atmoscudapropagator.cuh:42-145) and
-//                                     keeps AtmosCpuPropagator<FLOAT_T>'s API
-//                                     (atmoscpupropagator.hpp:35-547);
-//   HipBeamPropagatorSingle<FLOAT_T>  replaces BeamCudaPropagatorSingle
+//   Propagator<FLOAT_T>               the abstract base with the virtual set of
+//                                     the reference's Propagator<FLOAT_T>
+//                                     (propagator.hpp:148-252);
+//   AtmosPropagator<FLOAT_T>          the atmospheric interface: the virtual
+//                                     members AtmosCpuPropagator<FLOAT_T> adds
+//                                     (atmoscpupropagator.hpp:85-513);
+//   BeamPropagator<FLOAT_T>           the beam interface: the virtual members
+//                                     BeamCpuPropagator<FLOAT_T> adds
+//                                     (beamcpupropagator.hpp:84-140);
+//   HipPropagatorSingle<FLOAT_T>      : AtmosPropagator<FLOAT_T>, replaces
+//                                     AtmosCudaPropagatorSingle
+//                                     (atmoscudapropagator.cuh:42-145);
+//   HipBeamPropagatorSingle<FLOAT_T>  : BeamPropagator<FLOAT_T>, replaces
+//                                     BeamCudaPropagatorSingle
 //                                     (beamcudapropagator.cuh:41-121).
+//
+// A caller that holds a Propagator<double> (or AtmosPropagator<double>)
+// pointer and picks the concrete class at run time can take either class;
+// include/cudaprob3_amd/cudaprob3linear.hpp maps the reference's names onto
+// these.
 //
 // FLOAT_T = double computes in FP64 and FLOAT_T = float in FP32 by default;
 // Precision::Mixed (FP64 eigen-solve and phases, FP32 amplitude work) can be
@@ -54,8 +68,75 @@ std::vector<double> to_double(const std::vector<FLOAT_T>& v) {
 }
 }  // namespace detail
 
+// Propagator<FLOAT_T> (propagator.hpp:45-252): the members the reference
+// declares virtual, all pure here (the reference gives the first three a
+// host-side body that every concrete class here replaces with a device
+// upload).
 template <class FLOAT_T>
-class HipPropagatorSingle {
+class Propagator {
+ public:
+  virtual ~Propagator() {}
+  // propagator.hpp:148-183; delta is negated when kNuType < 0
+  virtual void setMNSMatrix(FLOAT_T theta12, FLOAT_T theta13, FLOAT_T theta23,
+                            FLOAT_T dCP, int kNuType) = 0;
+  // propagator.hpp:188-209; dm23sq is Delta m^2_32
+  virtual void setNeutrinoMasses(FLOAT_T dm12sq, FLOAT_T dm23sq) = 0;
+  // propagator.hpp:213-218
+  virtual void setEnergyList(const std::vector<FLOAT_T>& list) = 0;
+  // propagator.hpp:223
+  virtual void setChemicalComposition(const std::vector<FLOAT_T>& list) = 0;
+  // propagator.hpp:227
+  virtual void calculateProbabilities(NeutrinoType type) = 0;
+  // propagator.hpp:248
+  virtual void getProbabilityArr(FLOAT_T* probArr, ProbType t) = 0;
+  // propagator.hpp:250
+  virtual void setPathLength(FLOAT_T path_length) = 0;
+  // propagator.hpp:252
+  virtual void setDensity(FLOAT_T rho) = 0;
+};
+
+// The virtual members AtmosCpuPropagator<FLOAT_T> adds to the base
+// (atmoscpupropagator.hpp:85-513).
+template <class FLOAT_T>
+class AtmosPropagator : public Propagator<FLOAT_T> {
+ public:
+  using Propagator<FLOAT_T>::setDensity;
+  // atmoscpupropagator.hpp:85
+  virtual int getNlayerBoundaries() = 0;
+  // atmoscpupropagator.hpp:88-108
+  virtual void SetNumberOfProductionHeightBinsForAveraging(int n) = 0;
+  // atmoscpupropagator.hpp:116-170
+  virtual void setDensity(const std::vector<FLOAT_T>& radii, const std::vector<FLOAT_T>& rhos,
+                          const std::vector<FLOAT_T>& yps) = 0;
+  // atmoscpupropagator.hpp:181-245
+  virtual void setDensity(const std::vector<FLOAT_T>& radii, const std::vector<FLOAT_T>& a,
+                          const std::vector<FLOAT_T>& b, const std::vector<FLOAT_T>& c,
+                          const std::vector<FLOAT_T>& yps) = 0;
+  // atmoscpupropagator.hpp:250-344
+  virtual void setDensityFromFile(const std::string& filename) = 0;
+  // atmoscpupropagator.hpp:368-381
+  virtual void setCosineList(const std::vector<FLOAT_T>& list) = 0;
+  // atmoscpupropagator.hpp:386-393
+  virtual void setProductionHeight(FLOAT_T heightKM) = 0;
+  // atmoscpupropagator.hpp:395-427
+  virtual void setProductionHeightList(const std::vector<FLOAT_T>& list_prob,
+                                       const std::vector<FLOAT_T>& list_bins) = 0;
+  // atmoscpupropagator.hpp:431-444
+  virtual void setMaxlayers() = 0;
+  // atmoscpupropagator.hpp:492-500
+  virtual FLOAT_T getProbability(int index_cosine, int index_energy, ProbType t) = 0;
+};
+
+// The virtual members BeamCpuPropagator<FLOAT_T> adds to the base
+// (beamcpupropagator.hpp:123-130).
+template <class FLOAT_T>
+class BeamPropagator : public Propagator<FLOAT_T> {
+ public:
+  virtual FLOAT_T getProbability(int index_energy, ProbType t) = 0;
+};
+
+template <class FLOAT_T>
+class HipPropagatorSingle : public AtmosPropagator<FLOAT_T> {
   static_assert(std::is_floating_point<FLOAT_T>::value, "FLOAT_T must be float or double");
 
  public:
@@ -94,27 +175,27 @@ class HipPropagatorSingle {
 
   // propagator.hpp:148-183
   void setMNSMatrix(FLOAT_T theta12, FLOAT_T theta13, FLOAT_T theta23,
-                    FLOAT_T dCP, int kNuType) {
+                    FLOAT_T dCP, int kNuType) override {
     check(cp3_atmos_set_mns_matrix(live(), theta12, theta13, theta23, dCP, kNuType));
   }
   // propagator.hpp:188-209
-  void setNeutrinoMasses(FLOAT_T dm12sq, FLOAT_T dm23sq) {
+  void setNeutrinoMasses(FLOAT_T dm12sq, FLOAT_T dm23sq) override {
     check(cp3_atmos_set_neutrino_masses(live(), dm12sq, dm23sq));
   }
   // propagator.hpp:213-218
-  void setEnergyList(const std::vector<FLOAT_T>& list) {
+  void setEnergyList(const std::vector<FLOAT_T>& list) override {
     const std::vector<double> v = detail::to_double(list);
     check(cp3_atmos_set_energy_list(live(), v.data(), (int)v.size()));
   }
   // atmoscpupropagator.hpp:368-381
-  void setCosineList(const std::vector<FLOAT_T>& list) {
+  void setCosineList(const std::vector<FLOAT_T>& list) override {
     const std::vector<double> v = detail::to_double(list);
     check(cp3_atmos_set_cosine_list(live(), v.data(), (int)v.size()));
   }
   // atmoscpupropagator.hpp:116-170
   void setDensity(const std::vector<FLOAT_T>& radii,
                   const std::vector<FLOAT_T>& rhos,
-                  const std::vector<FLOAT_T>& yps) {
+                  const std::vector<FLOAT_T>& yps) override {
     if (rhos.size() != radii.size())
       throw std::runtime_error("setDensity : rhos.size() != radii.size()");
     if (rhos.size() != yps.size())
@@ -127,7 +208,7 @@ class HipPropagatorSingle {
   // averaged per trajectory segment.
   void setDensity(const std::vector<FLOAT_T>& radii, const std::vector<FLOAT_T>& a,
                   const std::vector<FLOAT_T>& b, const std::vector<FLOAT_T>& c,
-                  const std::vector<FLOAT_T>& yps) {
+                  const std::vector<FLOAT_T>& yps) override {
     if (a.size() != radii.size())
       throw std::runtime_error("setDensity : a.size() != radii.size()");
     if (a.size() != yps.size())
@@ -141,27 +222,27 @@ class HipPropagatorSingle {
                                      y.data(), (int)r.size()));
   }
   // atmoscpupropagator.hpp:347-357: a placeholder in the reference too.
-  void setDensity(FLOAT_T rho) {
+  void setDensity(FLOAT_T rho) override {
     (void)rho;
     std::cout << "DUMMY FUNCTION: ATMOS class uses setDensity(radii, rhos, yps) "
                  "or setDensityFromFile(filename)" << std::endl;
   }
   // atmoscpupropagator.hpp:250-344 (3- and 5-column files)
-  void setDensityFromFile(const std::string& filename) {
+  void setDensityFromFile(const std::string& filename) override {
     check(cp3_atmos_set_density_from_file(live(), filename.c_str()));
   }
   // atmoscpupropagator.hpp:481-490
-  void setChemicalComposition(const std::vector<FLOAT_T>& list) {
+  void setChemicalComposition(const std::vector<FLOAT_T>& list) override {
     const std::vector<double> v = detail::to_double(list);
     check(cp3_atmos_set_chemical_composition(live(), v.data(), (int)v.size()));
   }
   // atmoscpupropagator.hpp:386-393
-  void setProductionHeight(FLOAT_T heightKM) {
+  void setProductionHeight(FLOAT_T heightKM) override {
     check(cp3_atmos_set_production_height(live(), heightKM));
   }
   // atmoscpupropagator.hpp:88-108: at most 20 bins; like the reference, any
   // n >= 1 turns averaging off (setProductionHeightAveraging turns it on).
-  void SetNumberOfProductionHeightBinsForAveraging(int n) {
+  void SetNumberOfProductionHeightBinsForAveraging(int n) override {
     check(cp3_atmos_set_production_height_bins(live(), n));
     if (n >= 1) avg_ = false;
     if (avg_)
@@ -180,24 +261,24 @@ class HipPropagatorSingle {
   // [nutype][flavour][energy][cosine][bin] and the bin edges in km; throws
   // unless averaging is on.
   void setProductionHeightList(const std::vector<FLOAT_T>& list_prob,
-                               const std::vector<FLOAT_T>& list_bins) {
+                               const std::vector<FLOAT_T>& list_bins) override {
     const std::vector<double> p = detail::to_double(list_prob), b = detail::to_double(list_bins);
     check(cp3_atmos_set_production_height_list(live(), p.data(), (int64_t)p.size(), b.data(),
                                                (int)b.size()));
   }
   // atmoscpupropagator.hpp:361-364: a placeholder in the reference too.
-  void setPathLength(FLOAT_T) {
+  void setPathLength(FLOAT_T) override {
     std::cout << "DUMMY FUNCTION - ATMOS class calculates PATH LENGTH" << std::endl;
   }
   // atmoscpupropagator.hpp:85
-  int getNlayerBoundaries() {
+  int getNlayerBoundaries() override {
     int n = 0;
     check(cp3_atmos_get_nlayer_boundaries(live(), &n));
     return n;
   }
 
   // atmoscudapropagator.cu:202-205
-  void calculateProbabilities(NeutrinoType type) {
+  void calculateProbabilities(NeutrinoType type) override {
     check(cp3_atmos_calculate(live(), type == Antineutrino ? 1 : 0));
   }
   void calculateProbabilitiesAsync(NeutrinoType type) {
@@ -214,16 +295,16 @@ class HipPropagatorSingle {
   }
   // atmoscpupropagator.hpp:431-444: the shell counts are rebuilt with the
   // trajectory table whenever the cosines or the model change.
-  void setMaxlayers() {}
+  void setMaxlayers() override {}
 
   // atmoscudapropagator.cu:222-235
-  FLOAT_T getProbability(int index_cosine, int index_energy, ProbType t) {
+  FLOAT_T getProbability(int index_cosine, int index_energy, ProbType t) override {
     double v = 0;
     check(cp3_atmos_get_probability(live(), index_cosine, index_energy, (int)t, &v));
     return (FLOAT_T)v;
   }
   // atmoscudapropagator.cu:238-257: energy outer, cosine inner.
-  void getProbabilityArr(FLOAT_T* probArr, ProbType t) {
+  void getProbabilityArr(FLOAT_T* probArr, ProbType t) override {
     std::vector<double> tmp((size_t)n_cosines_ * (size_t)n_energies_);
     check(cp3_atmos_get_probability_arr(live(), tmp.data(), (int)t));
     for (size_t i = 0; i < tmp.size(); ++i) probArr[i] = (FLOAT_T)tmp[i];
@@ -255,7 +336,7 @@ class HipPropagatorSingle {
 };
 
 template <class FLOAT_T>
-class HipBeamPropagatorSingle {
+class HipBeamPropagatorSingle : public BeamPropagator<FLOAT_T> {
   static_assert(std::is_floating_point<FLOAT_T>::value, "FLOAT_T must be float or double");
 
  public:
@@ -290,29 +371,29 @@ class HipBeamPropagatorSingle {
   }
 
   void setMNSMatrix(FLOAT_T theta12, FLOAT_T theta13, FLOAT_T theta23,
-                    FLOAT_T dCP, int kNuType) {
+                    FLOAT_T dCP, int kNuType) override {
     check(cp3_beam_set_mns_matrix(live(), theta12, theta13, theta23, dCP, kNuType));
   }
-  void setNeutrinoMasses(FLOAT_T dm12sq, FLOAT_T dm23sq) {
+  void setNeutrinoMasses(FLOAT_T dm12sq, FLOAT_T dm23sq) override {
     check(cp3_beam_set_neutrino_masses(live(), dm12sq, dm23sq));
   }
-  void setEnergyList(const std::vector<FLOAT_T>& list) {
+  void setEnergyList(const std::vector<FLOAT_T>& list) override {
     const std::vector<double> v = detail::to_double(list);
     check(cp3_beam_set_energy_list(live(), v.data(), (int)v.size()));
   }
   // beamcpupropagator.hpp:84-91
-  void setDensity(FLOAT_T rho) { check(cp3_beam_set_density(live(), rho)); }
+  void setDensity(FLOAT_T rho) override { check(cp3_beam_set_density(live(), rho)); }
   // beamcpupropagator.hpp:112-120: a beam holds no per-row composition (its
   // Ye is 0.5, physics.hpp:1219), so only an empty list is accepted.
-  void setChemicalComposition(const std::vector<FLOAT_T>& list) {
+  void setChemicalComposition(const std::vector<FLOAT_T>& list) override {
     if (!list.empty())
       throw std::runtime_error(
           "BeamCpuPropagator::setChemicalComposition. Size of input list not equal to expectation");
   }
-  void setPathLength(FLOAT_T path_length) {
+  void setPathLength(FLOAT_T path_length) override {
     check(cp3_beam_set_path_length(live(), path_length));
   }
-  void calculateProbabilities(NeutrinoType type) {
+  void calculateProbabilities(NeutrinoType type) override {
     check(cp3_beam_calculate(live(), type == Antineutrino ? 1 : 0));
   }
   void calculateProbabilitiesAsync(NeutrinoType type) {
@@ -326,13 +407,13 @@ class HipBeamPropagatorSingle {
     check(cp3_beam_get_probability(live(), 0, 0, &v));
   }
   // beamcudapropagator.cu:146-159
-  FLOAT_T getProbability(int index_energy, ProbType t) {
+  FLOAT_T getProbability(int index_energy, ProbType t) override {
     double v = 0;
     check(cp3_beam_get_probability(live(), index_energy, (int)t, &v));
     return (FLOAT_T)v;
   }
   // beamcudapropagator.cu:162-178
-  void getProbabilityArr(FLOAT_T* probArr, ProbType t) {
+  void getProbabilityArr(FLOAT_T* probArr, ProbType t) override {
     std::vector<double> tmp((size_t)n_energies_);
     check(cp3_beam_get_probability_arr(live(), tmp.data(), (int)t));
     for (size_t i = 0; i < tmp.size(); ++i) probArr[i] = (FLOAT_T)tmp[i];

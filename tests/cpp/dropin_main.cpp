@@ -7,6 +7,8 @@
 //   dropin_main poly  <out.bin> <precision 0|1|2> <nutype 0|1>
 //   dropin_main avg   <out.bin> <precision 0|1|2> <nutype 0|1>
 //   dropin_main beam  <out.bin> <precision 0|1|2> <nutype 0|1>
+//   dropin_main virtual <out.bin> <precision 0|1|2> <nutype 0|1>
+//   dropin_main file  <out.bin> <precision 0|1|2> <nutype 0|1> <model.dat>
 //   dropin_main errors
 //
 // atmos: the 4-shell model, 100 x 100 grid (cos -1 + i/100, E 0.1-100 GeV
@@ -16,14 +18,24 @@
 // avg: the atmos case with production-height averaging over 5 bins (edges 5,
 // 10, 15, 22, 30, 50 km), bin probabilities height_prob() below.
 // beam: 1300 km, 2.6 g/cm^3, 500 energies over 0.1-20 GeV; output [E][9].
+// virtual: the atmos and beam cases through the reference's names
+// (cudaprob3linear.hpp) and base-class pointers only, the concrete class
+// picked at run time as a fitter would: std::unique_ptr<Propagator<double>>
+// for the calls of the reference's base (propagator.hpp:148-252) and
+// AtmosPropagator<double>& for the atmospheric setters; output the atmos
+// block [cos][E][9] then the beam block [E][9].
+// file: setDensityFromFile(model.dat) on a 20 x 20 grid (cos -1 + i/20,
+// E 1-100 GeV log), nominal parameters, h = 22 km; output [cos][E][9].
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "cudaprob3_amd/cudaprob3linear.hpp"
 #include "cudaprob3_amd/hip_propagator.hpp"
 
 using namespace cp3hip;
@@ -141,6 +153,108 @@ static int run_beam(const char* path, int precision, int nutype) {
   return 0;
 }
 
+// A fitter's factory: the concrete class behind a base pointer, chosen at
+// run time.
+static std::unique_ptr<cudaprob3linear::Propagator<double>> make_propagator(
+    bool beam, int n_cosines, int n_energies, int precision) {
+  if (beam)
+    return std::unique_ptr<cudaprob3linear::Propagator<double>>(
+        new cudaprob3linear::BeamCudaPropagatorSingle(0, n_energies, 1, (Precision)precision));
+  return std::unique_ptr<cudaprob3linear::Propagator<double>>(
+      new cudaprob3linear::AtmosCudaPropagatorSingle(0, n_cosines, n_energies,
+                                                     (Precision)precision));
+}
+
+// Every channel through the base's getProbabilityArr (energy outer, cosine
+// inner), re-ordered to [cos][E][9].
+static void all_channels(cudaprob3linear::Propagator<double>& p, int nC, int nE,
+                         double* out) {
+  std::vector<double> arr((size_t)nC * nE);
+  for (int t = 0; t < 9; ++t) {
+    p.getProbabilityArr(arr.data(), (cudaprob3linear::ProbType)t);
+    for (int e = 0; e < nE; ++e)
+      for (int c = 0; c < nC; ++c) out[((size_t)c * nE + e) * 9 + t] = arr[(size_t)e * nC + c];
+  }
+}
+
+static int run_virtual(const char* path, int precision, int nutype) {
+  const cudaprob3linear::NeutrinoType type =
+      nutype ? cudaprob3linear::Antineutrino : cudaprob3linear::Neutrino;
+  const int nC = 100, nE = 100, nB = 500;
+  std::vector<double> cz((size_t)nC);
+  for (int i = 0; i < nC; ++i) cz[(size_t)i] = -1.0 + i / 100.0;
+  std::vector<double> out((size_t)nC * nE * 9 + (size_t)nB * 9);
+
+  std::unique_ptr<cudaprob3linear::Propagator<double>> p =
+      make_propagator(false, nC, nE, precision);
+  cp3hip::AtmosPropagator<double>* atm = dynamic_cast<cp3hip::AtmosPropagator<double>*>(p.get());
+  if (!atm) return 5;
+  p->setEnergyList(logspace(0.1, 100.0, nE));
+  atm->setCosineList(cz);
+  p->setMNSMatrix(kTh12, kTh13, kTh23, kDcp, nutype ? -1 : 1);
+  p->setNeutrinoMasses(kDm12, kDm23);
+  atm->setDensity({0.0, 1220.0, 3480.0, 5701.0, 6371.0}, {13.0, 13.0, 11.3, 5.0, 3.3},
+                  {0.3, 0.3, 0.3, 0.3, 0.3});
+  // the composition through the base: Ye back to 0.5 on every row
+  p->setChemicalComposition({0.5, 0.5, 0.5, 0.5, 0.5});
+  atm->setProductionHeight(22.0);
+  p->calculateProbabilities(type);
+  all_channels(*p, nC, nE, out.data());
+  for (int c = 0; c < nC; c += 7)
+    for (int e = 0; e < nE; e += 11)
+      if (atm->getProbability(c, e, cudaprob3linear::m_e) != out[((size_t)c * nE + e) * 9 + 3]) {
+        std::fprintf(stderr, "getProbability through the interface differs\n");
+        return 3;
+      }
+
+  p = make_propagator(true, 0, nB, precision);
+  cp3hip::BeamPropagator<double>* bm = dynamic_cast<cp3hip::BeamPropagator<double>*>(p.get());
+  if (!bm) return 5;
+  p->setEnergyList(logspace(0.1, 20.0, nB));
+  p->setMNSMatrix(kTh12, kTh13, kTh23, kDcp, nutype ? -1 : 1);
+  p->setNeutrinoMasses(kDm12, kDm23);
+  p->setDensity(2.6);
+  p->setPathLength(1300.0);
+  p->calculateProbabilities(type);
+  double* bout = out.data() + (size_t)nC * nE * 9;
+  all_channels(*p, 1, nB, bout);
+  if (bm->getProbability(nB - 1, cudaprob3linear::e_m) != bout[(size_t)(nB - 1) * 9 + 1]) {
+    std::fprintf(stderr, "beam getProbability through the interface differs\n");
+    return 3;
+  }
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 2;
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+  std::printf("virtual ok\n");
+  return 0;
+}
+
+static int run_file(const char* path, int precision, int nutype, const char* model) {
+  const int nC = 20, nE = 20;
+  std::vector<double> cz((size_t)nC);
+  for (int i = 0; i < nC; ++i) cz[(size_t)i] = -1.0 + i / 20.0;
+  HipPropagatorSingle<double> p(0, nC, nE, (Precision)precision);
+  p.setEnergyList(logspace(1.0, 100.0, nE));
+  p.setCosineList(cz);
+  p.setMNSMatrix(kTh12, kTh13, kTh23, kDcp, nutype ? -1 : 1);
+  p.setNeutrinoMasses(kDm12, kDm23);
+  p.setDensityFromFile(model);
+  p.setProductionHeight(22.0);
+  p.calculateProbabilities(nutype ? Antineutrino : Neutrino);
+  std::vector<double> out((size_t)nC * nE * 9);
+  for (int c = 0; c < nC; ++c)
+    for (int e = 0; e < nE; ++e)
+      for (int t = 0; t < 9; ++t)
+        out[((size_t)c * nE + e) * 9 + t] = p.getProbability(c, e, (ProbType)t);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 2;
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+  std::printf("file ok: %d layer boundaries\n", p.getNlayerBoundaries());
+  return 0;
+}
+
 template <class Fn>
 static bool throws(Fn fn, const char* needle) {
   try {
@@ -171,6 +285,17 @@ static int run_errors() {
   p.setCosineList({-1.0, -0.5, 0.2});
   p.setEnergyList({1.0, 2.0, 3.0, 4.0});
   bad += !throws([&] { p.calculateProbabilities(Neutrino); }, "production height was not set");
+  {
+    // a model is needed before the first calculation (checked before any
+    // upload or launch)
+    HipPropagatorSingle<double> nomodel(0, 3, 4);
+    nomodel.setCosineList({-1.0, -0.5, 0.2});
+    nomodel.setEnergyList({1.0, 2.0, 3.0, 4.0});
+    nomodel.setMNSMatrix(kTh12, kTh13, kTh23, kDcp, 1);
+    nomodel.setNeutrinoMasses(kDm12, kDm23);
+    nomodel.setProductionHeight(15.0);
+    bad += !throws([&] { nomodel.calculateProbabilities(Neutrino); }, "density model was not set");
+  }
   p.setMNSMatrix(kTh12, kTh13, kTh23, kDcp, 1);
   p.setNeutrinoMasses(kDm12, kDm23);
   p.setDensity({0.0, 3480.0, 6371.0}, {11.3, 11.3, 4.0}, {0.5, 0.5, 0.5});
@@ -232,7 +357,8 @@ int main(int argc, char** argv) {
   try {
     if (argc >= 2 && std::string(argv[1]) == "errors") return run_errors();
     if (argc < 5) {
-      std::fprintf(stderr, "usage: %s atmos|poly|avg|beam out.bin precision nutype | errors\n",
+      std::fprintf(stderr,
+                   "usage: %s atmos|poly|avg|beam|virtual out.bin precision nutype | errors\n",
                    argv[0]);
       return 2;
     }
@@ -242,6 +368,8 @@ int main(int argc, char** argv) {
     if (mode == "poly") return run_atmos(argv[2], prec, nt, true);
     if (mode == "avg") return run_atmos(argv[2], prec, nt, false, true);
     if (mode == "beam") return run_beam(argv[2], prec, nt);
+    if (mode == "virtual") return run_virtual(argv[2], prec, nt);
+    if (mode == "file" && argc >= 6) return run_file(argv[2], prec, nt, argv[5]);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "exception: %s\n", e.what());
     return 4;

@@ -100,3 +100,62 @@ def test_dropin_beam_fp64(oracle, tmp_path, nutype):
 def test_dropin_errors():
     r = run("errors")
     assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (2, 1e-5)])
+@pytest.mark.parametrize("nutype", [0, 1])
+def test_dropin_through_base_pointers(oracle, tmp_path, nutype, prec, tol):
+    """Both classes behind std::unique_ptr<cudaprob3linear::Propagator<double>>
+    (the reference's names from cudaprob3linear.hpp), configured through the
+    base and the atmospheric/beam interfaces only, against the oracle."""
+    out = tmp_path / "v.bin"
+    assert run("virtual", out, prec, nutype).returncode == 0
+    got = np.fromfile(out, dtype=np.float64)
+    atm = got[:100 * 100 * 9].reshape(100, 100, 9)
+    beam = got[100 * 100 * 9:].reshape(500, 9)
+    e = W.prem4()
+    ref = oracle.atmos(nutype, W.cos_grid(100), W.log_grid(100, 0.1, 100.0),
+                       e.radii, e.rhos, e.yps, osc_for(NOM, nutype), 22.0,
+                       threads=4)
+    assert np.abs(atm - ref).max() < tol
+    refb = oracle.beam(nutype, W.log_grid(500, 0.1, 20.0), 2.6, 1300.0,
+                       osc_for(NOM, nutype))
+    assert np.abs(beam - refb).max() < tol
+
+
+def write_rows(path, rows):
+    """A density file laid out like models/PREM_4layer.dat (whitespace
+    separated rows, here with runs of spaces)."""
+    with open(path, "w") as f:
+        for r in rows:
+            f.write("   ".join(repr(float(x)) for x in r) + "\n")
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (2, 1e-5)])
+def test_dropin_three_column_file_vs_golden(tmp_path, prec, tol):
+    """setDensityFromFile on the rows of the reference's models/PREM_4layer.dat
+    (golden g5, written by the compiled reference from that file) through
+    the C++ class on the GPU."""
+    from conftest import golden
+    g = golden("g5_prem4_file")
+    model = tmp_path / "prem4.dat"
+    write_rows(model, g["model_rows"])
+    out = tmp_path / "f.bin"
+    r = run("file", out, prec, 0, model)
+    assert r.returncode == 0
+    assert "5 layer boundaries" in r.stdout
+    got = np.fromfile(out, dtype=np.float64).reshape(20, 20, 9)
+    err = np.abs(got - g["prob"]).max()
+    print("3-column file through the C++ class, precision", prec, "max|d|", err)
+    assert err < tol
+
+
+def test_dropin_two_column_file_refused(tmp_path):
+    """A 2-column (legacy) file is refused with an exception (the reference
+    ends the process, atmoscpupropagator.hpp:341)."""
+    model = tmp_path / "legacy.dat"
+    with open(model, "w") as f:
+        f.write("0. 13.0\n1220.0 13.0\n6371.0 3.3\n")
+    r = run("file", tmp_path / "x.bin", 0, 0, model)
+    assert r.returncode == 4
+    assert "exception" in r.stderr

@@ -473,3 +473,74 @@ def test_empty_grids_rejected():
             AtmosPropagator(nc, ne)
     with pytest.raises(Cp3Error, match="must be positive"):
         BeamPropagator(0)
+
+
+@pytest.mark.parametrize("mode", ["table", "shared"])
+def test_three_column_file_vs_golden(tmp_path, mode):
+    """setDensityFromFile with the rows of the reference's
+    models/PREM_4layer.dat written as a 3-column file (runs of spaces, a
+    trailing comment line): the compiled reference's result on that file
+    (golden g5) within 1e-10 in FP64 and 1e-5 mixed, and the same bits as
+    setDensity with the same rows; a 2-column file is refused.  As in the
+    reference (atmoscpupropagator.hpp:289-296), the rows of a 3-column file
+    are read as a stream of numbers up to the first non-number, so comment
+    lines may follow the rows but not precede them."""
+    from cudaprob3_amd import AtmosPropagator
+    from cudaprob3_amd._native import Cp3Error
+    g = golden("g5_prem4_file")
+    rows = g["model_rows"]
+    path = tmp_path / "prem4.dat"
+    with open(path, "w") as f:
+        for r in rows:
+            f.write("   ".join(repr(float(x)) for x in r) + "\n")
+        f.write("# radius density Ye\n")
+    t12, t13, t23, d, m12, m23 = NOM
+    for prec, tol in (("fp64", TOL_FP64), ("mixed", TOL_MIXED)):
+        with AtmosPropagator(20, 20, precision=prec) as p:
+            p.setSolveMode(mode)
+            p.setEnergyList(g["ev"])
+            p.setCosineList(g["cosv"])
+            p.setMNSMatrix(t12, t13, t23, d, 1)
+            p.setNeutrinoMasses(m12, m23)
+            p.setDensityFromFile(str(path))
+            p.setProductionHeight(22.0)
+            p.calculateProbabilities(0)
+            got = p.probabilities()
+            assert p.getNlayerBoundaries() == 5
+        err = np.abs(got - g["prob"]).max()
+        print("3-column file", prec, mode, "max|d| vs golden", err)
+        assert err < tol
+        arrays = run_atmos(g["cosv"], g["ev"], rows[:, 0], rows[:, 1], rows[:, 2],
+                           NOM, 0, prec, mode=mode)
+        assert np.array_equal(got, arrays)
+    legacy = tmp_path / "legacy.dat"
+    legacy.write_text("0. 13.0\n1220.0 13.0\n6371.0 3.3\n")
+    with AtmosPropagator(20, 20) as p:
+        with pytest.raises(Cp3Error, match="2 entries per line"):
+            p.setDensityFromFile(str(legacy))
+
+
+def test_calculate_without_model_refused():
+    """calculateProbabilities before any density model is a call-order error
+    (checked before any upload or kernel launch), and the object stays
+    usable: after setDensity it calculates normally."""
+    from cudaprob3_amd import AtmosPropagator
+    from cudaprob3_amd._native import Cp3Error, CP3_ERR_STATE
+    e = W.prem4()
+    cz, ev = W.cos_grid(8), W.log_grid(16, 0.1, 100.0)
+    t12, t13, t23, d, m12, m23 = NOM
+    for mode in ("table", "shared"):
+        with AtmosPropagator(len(cz), len(ev)) as p:
+            p.setSolveMode(mode)
+            p.setEnergyList(ev)
+            p.setCosineList(cz)
+            p.setMNSMatrix(t12, t13, t23, d, 1)
+            p.setNeutrinoMasses(m12, m23)
+            p.setProductionHeight(22.0)
+            with pytest.raises(Cp3Error, match="density model was not set") as ex:
+                p.calculateProbabilities(0)
+            assert ex.value.code == CP3_ERR_STATE
+            p.setDensity(e.radii, e.rhos, e.yps)
+            p.calculateProbabilities(0)
+            ref = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0)
+            assert np.array_equal(p.probabilities(), ref)
