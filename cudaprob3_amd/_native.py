@@ -59,6 +59,7 @@ _SIGS = {
     "cp3_atmos_get_nlayer_boundaries": (C.c_int, [_vp, _ip]),
     "cp3_atmos_calculate": (C.c_int, [_vp, C.c_int]),
     "cp3_atmos_calculate_async": (C.c_int, [_vp, C.c_int]),
+    "cp3_atmos_calculate_async_static": (C.c_int, [_vp, C.c_int, _vp]),
     "cp3_atmos_wait": (C.c_int, [_vp]),
     "cp3_atmos_get_probability": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp]),
     "cp3_atmos_get_probability_arr": (C.c_int, [_vp, _dp, C.c_int]),

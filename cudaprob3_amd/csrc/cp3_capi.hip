@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -91,6 +92,61 @@ int table_nc(int precision, const cp3::GridDev& g) {
   return 1;
 }
 
+// Scratch of cp3_atmos_calculate_async_static: the job descriptor (first
+// kScratchJobBytes) and the slab table of one calculation, in a static device
+// array instead of memory the object allocates.  Each device holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives with that device current (the array's name is a
+// host-side stand-in, not a device address).
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+constexpr size_t kScratchJobBytes = 512;
+static_assert(sizeof(cp3::AtmosJob) <= kScratchJobBytes, "job descriptor exceeds its slot");
+constexpr int kScratchDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct ScratchTable {
+  std::mutex mu;
+  bool resolved[kScratchDevices] = {};
+  void* ptr[kScratchDevices] = {};
+  hipError_t err[kScratchDevices] = {};
+};
+
+ScratchTable& scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+
+// Resolves `device`'s copy of the array once, with that device current.
+void resolve_scratch(int device) {
+  if (device < 0 || device >= kScratchDevices) return;
+  ScratchTable& t = scratch_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  if (t.resolved[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  t.err[device] = e;
+  t.ptr[device] = e == hipSuccess ? ptr : nullptr;
+  t.resolved[device] = true;
+  if (e != hipSuccess) (void)hipGetLastError();
+}
+
+// The device address of `device`'s copy (resolved by cp3_atmos_create).
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kScratchDevices)
+    throw Error(CP3_ERR_DEVICE, "static scratch supports devices 0.." +
+                                    std::to_string(kScratchDevices - 1));
+  ScratchTable& t = scratch_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  if (!t.resolved[device])
+    throw Error(CP3_ERR_STATE, "static scratch of device " + std::to_string(device) +
+                                   " was not resolved");
+  if (!t.ptr[device])
+    throw Error(CP3_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                    hipGetErrorString(t.err[device]));
+  return t.ptr[device];
+}
+
 }  // namespace
 
 struct cp3_atmos {
@@ -100,7 +156,9 @@ struct cp3_atmos {
   DevBuf d_slabs, d_job;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool launched = false;
+  // launched: a calculation wrote d_out; stream_used: some calculation
+  // (into d_out or a caller's buffer) may still read the device tables
+  bool launched = false, stream_used = false;
   std::vector<double> energies, cosines;
   bool cos_set = false, height_set = false;
   double h_cm = 0.0;
@@ -122,6 +180,8 @@ struct cp3_atmos {
 
   ~cp3_atmos() {
     DeviceScope ds(dev);
+    // a calculation in flight still reads the tables freed below
+    if (stream) (void)hipStreamSynchronize(stream);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -134,7 +194,7 @@ struct cp3_atmos {
   void upload_tables() {
     // the uploads are synchronous copies outside this object's stream, which
     // an asynchronous calculation may still be reading
-    if (launched && (energies_dirty || table_dirty))
+    if (stream_used && (energies_dirty || table_dirty))
       hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     if (energies_dirty) {
       d_ev.upload(energies.data(), energies.size() * sizeof(double), dev);
@@ -176,7 +236,8 @@ struct cp3_atmos {
     return g;
   }
 
-  void launch(int nutype) {
+  // The call-order rules of calculateProbabilities.
+  void check_ready() const {
     if (!height_set)
       throw Error(CP3_ERR_STATE,
                   "CudaPropagatorSingle::calculateProbabilities. production height was not set");
@@ -192,18 +253,47 @@ struct cp3_atmos {
       throw Error(CP3_ERR_STATE,
                   "CpuPropagator::calculateProbabilities. production height array was not set, "
                   "but been requested to use production height averaging");
+  }
+
+  // Whether a calculation runs the slab-table pre-pass (and so needs a
+  // table and a device job descriptor): not for the shared solve mode, nor
+  // for a polynomial model, whose densities differ per cosine (the shared
+  // solve with one cosine per wave and one wave per workgroup).
+  bool uses_table() const { return !(solve_mode == CP3_SOLVE_SHARED || rye_cs != 0); }
+  size_t table_bytes() const {
+    return (size_t)cp3::slab_entries(grid()) * cp3::slab_entry_bytes(precision);
+  }
+
+  void launch(int nutype) {
+    check_ready();
     DeviceScope ds(dev);
     upload_tables();
+    void* slabs = nullptr;
+    cp3::AtmosJob* dj = nullptr;
+    if (uses_table()) {
+      d_slabs.alloc(table_bytes(), dev);
+      d_job.alloc(sizeof(cp3::AtmosJob), dev);
+      slabs = d_slabs.p;
+      dj = (cp3::AtmosJob*)d_job.p;
+    }
+    results_on_host = false;
+    enqueue(nutype, d_out.p, slabs, dj, true);
+    launched = true;
+  }
+
+  // The kernels of one calculation on the object's stream, writing `out`
+  // ([9][nC][nE]); in table mode the pre-pass fills `slabs` and stores the
+  // job descriptor at `dj` for the propagation kernel.  `timed`: HIP events
+  // around the kernels.  Allocates nothing.
+  void enqueue(int nutype, void* out, void* slabs, cp3::AtmosJob* dj, bool timed) {
     cp3::AtmosJob job{};
     job.sc = osc.make(nutype);
     job.g = grid();
-    job.out = d_out.p;
+    job.out = out;
     job.unit_begin = 0;
     job.etiles = cp3::energy_tiles(job.g);
-    results_on_host = false;
-    // a polynomial model's densities differ per cosine: the shared solve with
-    // one cosine per wave and one wave per workgroup solves per trajectory
-    const bool per_cosine = job.g.rye_cs != 0;
+    const bool table = uses_table();
+    if (timed) hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
     if (avg) {
       // the bin probabilities of this neutrino type, one cosine per wave
       const int64_t plane = (int64_t)nC * nE;
@@ -211,50 +301,66 @@ struct cp3_atmos {
                   (size_t)nutype * 3 * n_bins * plane * cp3::out_elem_bytes(precision);
       job.pha.hmw = (const double*)d_pha_hmw.p;
       job.pha.nb = n_bins;
-      if (solve_mode == CP3_SOLVE_SHARED || per_cosine) {
-        hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+      if (!table) {
         hip_check(cp3::launch_atmos_single_avg(precision, job, stream),
                   "atmospheric kernel launch");
       } else {
-        d_slabs.alloc((size_t)cp3::slab_entries(job.g) * cp3::slab_entry_bytes(precision),
-                      dev);
-        job.slabs = d_slabs.p;
-        d_job.alloc(sizeof(job), dev);
-        cp3::AtmosJob* dj = (cp3::AtmosJob*)d_job.p;
+        job.slabs = slabs;
         const int64_t units = cp3::cosine_blocks(job.g, 1, 1) * job.etiles;
-        hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
         hip_check(cp3::launch_slab_table_single(precision, job, dj, stream),
                   "slab table launch");
         hip_check(cp3::launch_atmos_table_avg(precision, dj, units,
                                               (int)((units + 7) / 8 * 8), stream),
                   "atmospheric kernel launch");
       }
-    } else if (solve_mode == CP3_SOLVE_SHARED || per_cosine) {
+    } else if (!table) {
       int nc = 1, w = 1;
       auto_tuning(precision, job.g, &nc, &w);
       if (nc_force) nc = nc_force;
       if (w_force) w = w_force;
-      if (per_cosine) nc = w = 1;
-      hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+      if (rye_cs != 0) nc = w = 1;
       hip_check(cp3::launch_atmos_single(precision, nc, w, job, stream),
                 "atmospheric kernel launch");
     } else {
       // the grid's slab table (a pre-pass) read by the propagation kernel
       const int nc = nc_force ? nc_force : table_nc(precision, job.g);
-      d_slabs.alloc((size_t)cp3::slab_entries(job.g) * cp3::slab_entry_bytes(precision), dev);
-      job.slabs = d_slabs.p;
-      d_job.alloc(sizeof(job), dev);
-      cp3::AtmosJob* dj = (cp3::AtmosJob*)d_job.p;
+      job.slabs = slabs;
       const int64_t units = cp3::cosine_blocks(job.g, nc, 1) * job.etiles;
-      hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
       hip_check(cp3::launch_slab_table_single(precision, job, dj, stream),
                 "slab table launch");
       hip_check(cp3::launch_atmos_table(precision, nc, dj, 1, units,
                                         (int)((units + 7) / 8 * 8), stream),
                 "atmospheric kernel launch");
     }
-    hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
-    launched = true;
+    if (timed) hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+    stream_used = true;
+  }
+
+  // cp3_atmos_calculate_async_static: the same kernels into the caller's
+  // device buffer, with the table and job descriptor in the static scratch.
+  void launch_static(int nutype, void* out) {
+    check_ready();
+    if (energies_dirty || table_dirty)
+      throw Error(CP3_ERR_STATE,
+                  "cp3_atmos_calculate_async_static: the energies, cosines, Earth model or "
+                  "production height changed since the last calculation; a "
+                  "cp3_atmos_calculate(_async) call uploads them first");
+    DeviceScope ds(dev);
+    void* slabs = nullptr;
+    cp3::AtmosJob* dj = nullptr;
+    if (uses_table()) {
+      const size_t need = kScratchJobBytes + table_bytes();
+      if (need > kScratchBytes)
+        throw Error(CP3_ERR_INVALID,
+                    "cp3_atmos_calculate_async_static: this grid's slab table needs " +
+                        std::to_string(need) + " bytes of scratch, more than the " +
+                        std::to_string(kScratchBytes) +
+                        " bytes of the static array (cp3_atmos_calculate_async has no limit)");
+      char* base = (char*)scratch_ptr(dev);
+      dj = (cp3::AtmosJob*)base;
+      slabs = base + kScratchJobBytes;
+    }
+    enqueue(nutype, out, slabs, dj, false);
   }
 
   void wait() {
@@ -398,6 +504,8 @@ int cp3_atmos_create(int device, int n_cosines, int n_energies, int precision,
     hip_check(hipEventCreate(&p->ev1), "hipEventCreate");
     p->d_out.alloc(p->out_bytes(), device);
     p->h_out.alloc(p->out_bytes());
+    // this device's copy of the static scratch (cp3_atmos_calculate_async_static)
+    resolve_scratch(device);
     *out = p.release();
   });
 }
@@ -614,6 +722,15 @@ int cp3_atmos_calculate_async(cp3_atmos* p, int nutype) {
     need(p);
     check_nutype(nutype);
     p->launch(nutype);
+  });
+}
+
+int cp3_atmos_calculate_async_static(cp3_atmos* p, int nutype, void* d_out) {
+  return guard([&] {
+    need(p);
+    check_nutype(nutype);
+    need_ptr(d_out, "d_out");
+    p->launch_static(nutype, d_out);
   });
 }
 

@@ -146,6 +146,26 @@ int cp3_atmos_calculate(cp3_atmos* p, int nutype);
  * wait (waitForCompletion, .cu:313-316). */
 int cp3_atmos_calculate_async(cp3_atmos* p, int nutype);
 int cp3_atmos_wait(cp3_atmos* p);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc): cp3_atmos_calculate_async with nothing allocated.
+ * The kernels are the same and so are the results (bitwise); they are
+ * written to the caller's device buffer d_out ([9][nC][nE], double for
+ * CP3_FP64, float otherwise), not to the object's own result, which
+ * getProbability keeps reading.  The slab table and job descriptor of the
+ * default solve mode live in a static device array of 16 MiB per GPU
+ * (512 B for the descriptor, the rest for the table: (shells + 1) x
+ * (nE rounded up to 64) x 40 B in mixed and FP32 precision, 64 B in FP64);
+ * a grid whose table exceeds it returns CP3_ERR_INVALID, before any launch.
+ * The object's device inputs must be current: after a setter that changed
+ * the energies, cosines, Earth model or production height, one
+ * cp3_atmos_calculate(_async) uploads them, and until then this returns
+ * CP3_ERR_STATE.  Launches on the object's stream (cp3_atmos_get_stream)
+ * and records no timing events; cp3_atmos_wait waits.  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (different
+ * objects, or streams not ordered by the caller) race and must be
+ * serialized. */
+int cp3_atmos_calculate_async_static(cp3_atmos* p, int nutype, void* d_out);
 
 /* getProbability(ic, ie, t) (atmoscudapropagator.cu:222-235): the first
  * access after a calculate copies the whole result to pinned host memory. */
