@@ -169,34 +169,60 @@ def kernel_name(precision, tuning):
     return f"atmos_batch_kernel<{pol}, {nc}, {w}>"
 
 
-def cpu_baseline(wl, gpu_batch, budget_s=12.0):
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def host_cores():
+    """Cores this process may run on, and the OpenMP thread count: those
+    cores capped at 16, the host-core share of one GPU of an 8-GPU MI355X
+    node (the affinity mask can show the whole node's cores)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return n, max(1, min(n, 16))
+
+
+def cpu_baseline(wl, gpu_batch, budget_s=12.0, budget_1t_s=4.0):
     """The reference CPU propagator (compiled from the reference headers into
     oracle/_ref, or the oracle restatement where that was not built), timed on
-    a bounded sample of the same jobs with all host cores given to it; the
-    GPU results of the same jobs are compared to its output."""
+    a bounded sample of the same jobs: with the host cores given to it
+    (OpenMP over cosines, physics.hpp:828-830), and on one thread; the rates
+    are the sample's evaluations over its time, so the whole batch's figure is
+    extrapolated linearly in jobs.  The GPU results of the sampled jobs are
+    compared to its output."""
     sys.path.insert(0, ROOT)
     from oracle.loader import Osc, load_oracle, load_ref
     lib = load_ref(deep=True)
     kind = "reference"
     if lib is None:
         lib, kind = load_oracle(), "port"
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    visible, cores = host_cores()
     order = []  # alternate big/small grids and nu/nubar
     half = len(wl.jobs) // 2
     for i in range(0, half, 2):
         order += [i, i + 1, half + i, half + i + 1]
+
+    def osc_of(job):
+        t12, t13, t23, d, k, m12, m23 = job.params.for_nutype(job.nutype)
+        return Osc(t12, t13, t23, d, m12, m23, k)
+
     evals, elapsed, worst, sq, npts, used = 0, 0.0, 0.0, 0.0, 0, 0
     for j in order:
         job = wl.jobs[j]
-        t12, t13, t23, d, k, m12, m23 = job.params.for_nutype(job.nutype)
-        osc = Osc(t12, t13, t23, d, m12, m23, k)
         t0 = time.perf_counter()
         ref = lib.atmos(job.nutype, job.cosv, job.ev, wl.earth.radii,
-                        wl.earth.rhos, wl.earth.yps, osc, wl.h_km,
+                        wl.earth.rhos, wl.earth.yps, osc_of(job), wl.h_km,
                         threads=cores)
         elapsed += time.perf_counter() - t0
         evals += ref.shape[0] * ref.shape[1]
@@ -208,11 +234,33 @@ def cpu_baseline(wl, gpu_batch, budget_s=12.0):
         used += 1
         if elapsed >= budget_s:
             break
+    # one thread, on the same job order
+    evals1, elapsed1, used1 = 0, 0.0, 0
+    for j in order:
+        job = wl.jobs[j]
+        t0 = time.perf_counter()
+        ref = lib.atmos(job.nutype, job.cosv, job.ev, wl.earth.radii,
+                        wl.earth.rhos, wl.earth.yps, osc_of(job), wl.h_km,
+                        threads=1)
+        elapsed1 += time.perf_counter() - t0
+        evals1 += ref.shape[0] * ref.shape[1]
+        used1 += 1
+        if elapsed1 >= budget_1t_s:
+            break
     return {
         "value": evals / elapsed, "unit": "evals/s", "cores": cores,
         "kind": kind,
         "sample": f"{used} of {len(wl.jobs)} jobs ({evals} evals, "
                   f"{elapsed:.1f} s, AtmosCpuPropagator<double> with OpenMP)",
+        "extrapolated": True,
+        "sampled_jobs": used,
+        "total_jobs": len(wl.jobs),
+        "cpu_model": cpu_model(),
+        "visible_cores": visible,
+        "cores_note": "OpenMP threads = the cores this process may use, at most "
+                      "16 (the host-core share of one GPU of an 8-GPU node)",
+        "one_thread": {"value": evals1 / elapsed1, "unit": "evals/s",
+                       "sample": f"{used1} jobs ({evals1} evals, {elapsed1:.1f} s)"},
     }, worst, float(np.sqrt(sq / max(npts, 1)))
 
 

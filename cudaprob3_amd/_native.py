@@ -11,7 +11,9 @@ import os
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 HIP_SO = os.path.join(PKG, "libcp3hip.so")
-MODEL_SO = os.path.join(PKG, "libcp3model.so")
+# CP3_MODEL_SO selects another build of the host model library (the
+# sanitizer build, libcp3model_asan.so, for the CPU tests)
+MODEL_SO = os.environ.get("CP3_MODEL_SO") or os.path.join(PKG, "libcp3model.so")
 
 CP3_FP64, CP3_FP32, CP3_MIXED = 0, 1, 2
 CP3_SOLVE_AUTO, CP3_SOLVE_SHARED, CP3_SOLVE_TABLE = 0, 1, 2

@@ -618,13 +618,19 @@ __global__ void __launch_bounds__(kWave) beam_single_kernel(BeamJob job) {
   beam_unit<P>(job, (int64_t)blockIdx.x, (int)threadIdx.x);
 }
 
-// 128 energies of one job per wave, two per lane (energies e and e + 64: two
-// independent solves in flight), kBeamWaves waves per workgroup.  Without a
-// loop the solve's constants are not held in registers across units, which
-// keeps the kernel at 46-73 VGPRs (6-8 waves per SIMD).  The grid is 2-D (y:
-// job, x: groups of kBeamWaves 128-energy units), so a wave finds its job
-// without a search; waves past a shorter job's energies exit at once.
+// kBeamPerLane energies of one job per lane (e, e + 64, ...: independent
+// solves in flight), so a wave covers kBeamPerLane * 64 consecutive energies,
+// and kBeamWaves waves per workgroup.  Without a loop the solve's constants
+// are not held in registers across units.  The grid is 2-D (y: job, x: groups
+// of kBeamWaves waves), so a wave finds its job without a search; waves past
+// a shorter job's energies exit at once.  The kernel is VALU-bound: on the
+// 4096-set x 1e5-energy batch (MI355X) the same arithmetic without the
+// stores takes 73-84% of the time of the kernel with two energies per lane
+// (FP64 to FP32 forms), the stores alone 60% (FP32 form, 5.6 TB/s); four
+// energies per lane overlap the two better (FP32 4.41 -> 3.86 ms, mixed
+// 6.50 -> 6.22 ms, FP64 8.36 -> 8.12 ms).
 constexpr int kBeamWaves = 4;
+constexpr int kBeamPerLane = 4;
 template <class P>
 __global__ void __launch_bounds__(kWave * kBeamWaves) beam_batch_kernel(
     const BeamJob* __restrict__ jobs, int job0) {
@@ -634,24 +640,23 @@ __global__ void __launch_bounds__(kWave * kBeamWaves) beam_batch_kernel(
   const BeamJob& job = jobs[job0 + (int)blockIdx.y];
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
   const int64_t local = (int64_t)blockIdx.x * kBeamWaves + wave;
-  if (local * 2 * kWave >= job.nE) return;
+  if (local * kBeamPerLane * kWave >= job.nE) return;
   const int lane = (int)threadIdx.x % kWave;
-  const int e0 = (int)local * 2 * kWave + lane, e1 = e0 + kWave;
-  const bool ok0 = e0 < job.nE, ok1 = e1 < job.nE;
-  const D E0 = (D)job.ev[ok0 ? e0 : job.nE - 1];
-  const D E1 = (D)job.ev[ok1 ? e1 : job.nE - 1];
-  F p0[9], p1[9];
-  beam_point<P>(job.sc, E0, (D)job.rye, (D)job.L, p0);
-  beam_point<P>(job.sc, E1, (D)job.rye, (D)job.L, p1);
   O* out = (O*)job.out;
-  if (ok0) {
+  F p[kBeamPerLane][9];
+  int e[kBeamPerLane];
 #pragma unroll
-    for (int ch = 0; ch < 9; ++ch) out[(int64_t)ch * job.nE + e0] = (O)p0[ch];
+  for (int k = 0; k < kBeamPerLane; ++k) {
+    e[k] = (int)local * kBeamPerLane * kWave + k * kWave + lane;
+    const D E = (D)job.ev[e[k] < job.nE ? e[k] : job.nE - 1];
+    beam_point<P>(job.sc, E, (D)job.rye, (D)job.L, p[k]);
   }
-  if (ok1) {
 #pragma unroll
-    for (int ch = 0; ch < 9; ++ch) out[(int64_t)ch * job.nE + e1] = (O)p1[ch];
-  }
+  for (int k = 0; k < kBeamPerLane; ++k)
+    if (e[k] < job.nE) {
+#pragma unroll
+      for (int ch = 0; ch < 9; ++ch) out[(int64_t)ch * job.nE + e[k]] = (O)p[k][ch];
+    }
 }
 
 // ------------------------------------------------------------- launchers
@@ -905,8 +910,9 @@ hipError_t launch_beam_batch(int precision, const BeamJob* jobs, int njobs,
   if (max_job_units > 0x7fffffff) return hipErrorInvalidConfiguration;
   // at most 65535 jobs per launch (grid y)
   for (int j0 = 0; j0 < njobs; j0 += 65535) {
-    const int64_t pairs = (max_job_units + 1) / 2;  // 128-energy units
-    const dim3 g((unsigned)((pairs + kBeamWaves - 1) / kBeamWaves),
+    // waves of kBeamPerLane * 64 energies
+    const int64_t waves = (max_job_units + kBeamPerLane - 1) / kBeamPerLane;
+    const dim3 g((unsigned)((waves + kBeamWaves - 1) / kBeamWaves),
                  (unsigned)std::min(njobs - j0, 65535)),
         b(kWave * kBeamWaves);
     switch (precision) {

@@ -748,17 +748,68 @@ CP3_HD void traj_finish_avg(const Traj<P>& t, const Slab<P>& vac,
 }
 
 // A single constant-density layer of length L (the beam path,
-// physics.hpp:1153-1260): Z = I, then the same finish as a trajectory.
+// physics.hpp:1153-1260).  In the real frame the layer matrix is
+// T = I + sum_k w_k v_k v_k^T over the two non-reference eigenvectors, and
+// the flavour-basis amplitude is B = W T W^+ with W = R23 G, so
+//   B = I + sum_k w_k u_k u_k^+,  u_k = W v_k = (v0, c v1 + s e^{id} v2,
+//                                               -s v1 + c e^{id} v2),
+// and P(a->b) = |B[b][a]|^2.  The Hermitian products u_k[b] conj(u_k[a])
+// serve both B[b][a] and B[a][b] (the latter takes their conjugates), so a
+// pair of entries costs one set of products.
 template <class P>
 CP3_HD void beam_point(const SetConst& sc, typename P::D E,
                        typename P::D rhoYe, typename P::D L,
                        typename P::F prob[9]) {
   using D = typename P::D;
+  using F = typename P::F;
   Slab<P> s;
   solve_slab<P>(sc, E, D(1) / E, rhoYe, s);
-  Traj<P> t;
-  traj_init<P>(t, true);
-  traj_finish<P>(t, s, L, sc, prob);
+  const F c = F(sc.c23), sn = F(sc.s23), cd = F(sc.cd), sd = F(sc.sd);
+  F wr[2], wi[2], ur[2][3], ui[2][3];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const F* v = k == 0 ? s.va : s.vb;
+    Phase<P>::eval(k == 0 ? s.oma : s.omb, L, wr[k], wi[k]);
+    const F g2r = cd * v[2], g2i = sd * v[2];
+    ur[k][0] = v[0];
+    ui[k][0] = F(0);
+    ur[k][1] = c * v[1] + sn * g2r;
+    ui[k][1] = sn * g2i;
+    ur[k][2] = c * g2r - sn * v[1];
+    ui[k][2] = c * g2i;
+  }
+  // diagonal: B[m][m] = 1 + sum_k w_k |u_k[m]|^2
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    F br = F(1), bi = F(0);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const F d = ur[k][m] * ur[k][m] + ui[k][m] * ui[k][m];
+      br = br + wr[k] * d;
+      bi = bi + wi[k] * d;
+    }
+    prob[m * 3 + m] = br * br + bi * bi;
+  }
+  // pairs (b, a) = (1, 0), (2, 0), (2, 1): h_k = u_k[b] conj(u_k[a]),
+  // B[b][a] = sum_k w_k h_k, B[a][b] = sum_k w_k conj(h_k)
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const int b = p == 0 ? 1 : 2, a = p == 2 ? 1 : 0;
+    F x = F(0), y = F(0), z = F(0), q = F(0);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const F hr = ur[k][b] * ur[k][a] + ui[k][b] * ui[k][a];
+      const F hi = ui[k][b] * ur[k][a] - ur[k][b] * ui[k][a];
+      x = x + wr[k] * hr;
+      y = y + wi[k] * hi;
+      z = z + wr[k] * hi;
+      q = q + wi[k] * hr;
+    }
+    const F bar = x - y, bai = z + q;  // B[b][a]
+    const F abr = x + y, abi = q - z;  // B[a][b]
+    prob[a * 3 + b] = bar * bar + bai * bai;
+    prob[b * 3 + a] = abr * abr + abi * abi;
+  }
 }
 
 }  // namespace cp3

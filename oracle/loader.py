@@ -19,7 +19,9 @@ from dataclasses import dataclass
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# CP3_ORACLE_SO selects another build of the oracle (the sanitizer build,
+# liboracle_asan.so)
+ORACLE_SO = os.environ.get("CP3_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcp3ref.so")
 REF_DEEP_SO = os.path.join(HERE, "_ref", "libcp3ref_deep.so")
 
