@@ -352,6 +352,19 @@ def main():
                 "frac": rate / cap, "clock_hz_assumed": CLOCK_HZ,
                 "detail": valu_rec}
 
+    # the same rate from rocprofv3's VALU FLOP counters of this kernel on this
+    # workload (instruction counts, committed), over this run's launch time
+    flop_rec = FL.measured_flops(dom)
+    counters = None
+    if flop_rec:
+        ach_c = (flop_rec["f64_flop"] + flop_rec["f32_flop"] * PEAK_F64 / PEAK_F32) / t_dom
+        counters = {"achieved": ach_c / 1e12, "frac": ach_c / PEAK_F64,
+                    "f64_flop": flop_rec["f64_flop"], "f32_flop": flop_rec["f32_flop"],
+                    "counted_with_phase_polynomials": (
+                        FL.with_phase_polynomials(counts)
+                        if tune_m[3] == "table" else None),
+                    "source": flop_rec["source"]}
+
     result = {
         "metric": METRIC,
         "value": value,
@@ -383,19 +396,28 @@ def main():
         "roofline": {
             "kernel": dom,
             "bound": "valu",
-            "achieved": achieved / 1e12,
+            # rocprofv3's VALU FLOP counters of this kernel where recorded,
+            # else the counted FLOPs
+            "achieved": (counters["achieved"] if counters else achieved / 1e12),
             "peak": PEAK_F64 / 1e12,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_F64,
+            "frac": (counters["frac"] if counters else achieved / PEAK_F64),
+            "achieved_from": "rocprofv3 VALU FLOP counters" if counters else "counted FLOPs",
+            "achieved_counted": achieved / 1e12,
+            "frac_counted": achieved / PEAK_F64,
             "traffic": traffic,
             "traffic_detail": traffic_rec,
             "valu_issue": valu,
+            "counters_detail": counters,
             "kernel_ms": phases["atmos"],
-            "note": "FP64-equivalent rate (F64 + F32*78.6/157.3) of the "
-                    "counted FLOPs of the exact kernel decomposition over the "
-                    "atmospheric propagation kernel's own launch time "
-                    "(HIP events between the launches of eager runs of the "
-                    "timed batch); transcendentals counted separately",
+            "note": "FP64-equivalent rate (F64 + F32*78.6/157.3) over the "
+                    "atmospheric propagation kernel's own launch time (HIP "
+                    "events between the launches of eager runs of the timed "
+                    "batch); achieved: FLOPs from the VALU instruction counters "
+                    "(64 x (ADD + MUL + 2 FMA) per type, committed profile of "
+                    "this kernel and workload); achieved_counted: the counted "
+                    "FLOPs of the kernel's decomposition, layer-phase "
+                    "polynomials counted as sincos calls",
             "flops_f64": counts["f64"], "flops_f32": counts["f32"],
             "sincos_f64": counts["sincos64"], "sincos_f32": counts["sincos32"],
             "batch_achieved": batch_rate / 1e12,

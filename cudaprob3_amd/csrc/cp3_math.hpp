@@ -288,7 +288,9 @@ struct Slab {
 // product): the column j with the largest P_jj dk = dk^2 v_j^2,
 // dk = (lk - lp)(lk - lq), scaled by 1/sqrt(P_jj dk).  H and H2 = H^2 as
 // upper triangles 00 01 02 11 12 22.
-template <class D, class F>
+// kFNorm (beam path only): an F eigenvector of the mixed form is normalised
+// in F; the column itself, whose entries cancel, is still formed in D.
+template <class D, class F, bool kFNorm = false>
 CP3_HD void eigvec(const D H[6], const D H2[6], D lk, D lp, D lq, F v[3]) {
   const D sp = lp + lq, pp = lp * lq;
   const D p00 = (H2[0] - sp * H[0]) + pp;
@@ -304,10 +306,18 @@ CP3_HD void eigvec(const D H[6], const D H2[6], D lk, D lp, D lq, F v[3]) {
   const D c0 = j0 ? p00 : (j1 ? p01 : p02);
   const D c1 = j0 ? p01 : (j1 ? p11 : p12);
   const D c2 = j0 ? p02 : (j1 ? p12 : p22);
-  const D inv = m_rsqrt(j0 ? s0 : (j1 ? s1 : s2));
-  v[0] = (F)(c0 * inv);
-  v[1] = (F)(c1 * inv);
-  v[2] = (F)(c2 * inv);
+  const D sj = j0 ? s0 : (j1 ? s1 : s2);
+  if (kFNorm && sizeof(F) < sizeof(D)) {
+    const F inv = m_rsqrt((F)sj);
+    v[0] = (F)c0 * inv;
+    v[1] = (F)c1 * inv;
+    v[2] = (F)c2 * inv;
+  } else {
+    const D inv = m_rsqrt(sj);
+    v[0] = (F)(c0 * inv);
+    v[1] = (F)(c1 * inv);
+    v[2] = (F)(c2 * inv);
+  }
 }
 
 // Matter eigenvalues in the reference's order: the trigonometric roots of
@@ -336,8 +346,9 @@ CP3_HD void matter_eigs(const SetConst& sc, D fac, D lam[3]) {
     lam[k] = amp * (D(sc.ra[k]) * cs + D(sc.rb[k]) * sn) + shift;
 }
 
-// Solves one slab of electron density rhoYe at energy E (invE = 1/E).
-template <class P>
+// Solves one slab of electron density rhoYe at energy E (invE = 1/E);
+// kFNorm as for eigvec.
+template <class P, bool kFNorm = false>
 CP3_HD void solve_slab(const SetConst& sc, typename P::D E,
                        typename P::D invE, typename P::D rhoYe, Slab<P>& s) {
   using D = typename P::D;
@@ -360,8 +371,8 @@ CP3_HD void solve_slab(const SetConst& sc, typename P::D E,
   const D lr = p01 ? lam[0] : lam[2];
   const D ln = (p01 || p12) ? lam[1] : lam[0];
   const D lf = p01 ? lam[2] : (p12 ? lam[0] : lam[1]);
-  eigvec<D>(H, H2, lf, lr, ln, s.va);
-  eigvec<D>(H, H2, ln, lr, lf, s.vb);
+  eigvec<D, typename P::F, kFNorm>(H, H2, lf, lr, ln, s.va);
+  eigvec<D, typename P::F, kFNorm>(H, H2, ln, lr, lf, s.vb);
   const D k = D(-kLoE * Phase<P>::kOmScale) * invE;
   s.oma = k * (lf - lr);
   s.omb = k * (ln - lr);
@@ -763,7 +774,10 @@ CP3_HD void beam_point(const SetConst& sc, typename P::D E,
   using D = typename P::D;
   using F = typename P::F;
   Slab<P> s;
-  solve_slab<P>(sc, E, D(1) / E, rhoYe, s);
+  // the mixed form normalises the eigenvectors in F here: one layer adds no
+  // error of its own (1.2e-6 from FP64 on 0.1-20 GeV at 1300 km), where
+  // chained layers add up
+  solve_slab<P, true>(sc, E, D(1) / E, rhoYe, s);
   const F c = F(sc.c23), sn = F(sc.s23), cd = F(sc.cd), sd = F(sc.sd);
   F wr[2], wi[2], ur[2][3], ui[2][3];
 #pragma unroll

@@ -169,3 +169,35 @@ def measured_traffic(name: str):
     with open(path) as f:
         data = json.load(f)
     return data.get(name)
+
+
+def measured_flops(name: str):
+    """FLOPs per launch from rocprofv3's VALU instruction counters
+    (64 x (ADD + MUL + 2 FMA) of SQ_INSTS_VALU_{ADD,MUL,FMA}_{F32,F64}, plus
+    the hardware's own SQ_INSTS_VALU_FLOPS_FP32/FP64 and the TRANS counts),
+    as recorded in profiles/valu_flops.json, or None.  Unlike the counted
+    FLOPs above, these include the phase polynomials' multiply-adds (counted
+    above as sincos calls) and exclude work the compiler shares between a
+    lane's trajectories."""
+    path = os.path.join(PROFILES, "valu_flops.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    return data.get(name)
+
+
+# FLOPs of one layer-phase evaluation e^{2 pi i f} - 1 (cp3_math.hpp
+# m_cispi_m1), which the counts above record as one sincos call: z = f*f,
+# Horner chains of degree 9 and 10 in f (4 + 5 fused multiply-adds) in FP32 or
+# 15 and 16 (7 + 8) in FP64, then a = pa f, wr = -(a a), wi = a b.
+PHASE_POLY_FLOP = {"f32": 1 + 2 * 4 + 2 * 5 + 3, "f64": 1 + 2 * 7 + 2 * 8 + 3}
+
+
+def with_phase_polynomials(counts: dict) -> dict:
+    """Counted FLOPs of a propagation kernel with no eigen-solve inside (the
+    slab-table kernels), its phase polynomials included: every sincos call
+    there is a layer-phase evaluation.  This is the figure the VALU FLOP
+    counters measure."""
+    return {"f64": counts["f64"] + PHASE_POLY_FLOP["f64"] * counts["sincos64"],
+            "f32": counts["f32"] + PHASE_POLY_FLOP["f32"] * counts["sincos32"]}

@@ -125,6 +125,9 @@ def test_stage_counts():
     assert f64["middle"]["sincos64"] == 2 and mixed["middle"]["sincos32"] == 2
     assert f64["solve"]["acos64"] == 1 and mixed["solve"]["acos64"] == 1
     assert mixed["solve"]["f32"] == 0  # the eigen-solve stays FP64
+    # ... but for the beam's F eigenvectors, normalised in F: its FP64 work is
+    # below one solve plus its two phase reductions (q = om L, q - rint(q))
+    assert mixed["beam"]["f64"] < mixed["solve"]["f64"] + 2 * 2
     # the hoisting: fewer solves per trajectory with larger cosine blocks
     wl = W.atmos1024(n_sets=4)
     c1 = FL.batch_counts(wl, "mixed", 1)
