@@ -96,7 +96,8 @@ def other_configs():
     the same grid with production-height averaging over 20 bins (edges 5-50 km,
     equal bin probabilities; FP64, mixed; pre-pass plus averaging kernel), and
     the beam batch of 4096 seeded sets x 1e5 energies (0.1-20 GeV, 1300 km,
-    2.6 g/cm^3) in one mixed launch."""
+    2.6 g/cm^3) in one launch (mixed, the configuration's form, then FP32 and
+    FP64; median of 10 after one warm-up)."""
     from cudaprob3_amd import AtmosPropagator
     from cudaprob3_amd.batch import BEAM
     out = {}
@@ -145,18 +146,32 @@ def other_configs():
     out["height_avg_12shell_2000x2000_20bins"] = avg
     bev = W.log_grid(100000, 0.1, 20.0)
     sets = W.draw_param_sets(4096, 4096)
-    with Batch("mixed") as b:
-        g = b.add_beam_grid(bev, 2.6, 1300.0)
-        b.set_jobs([make_job(BEAM, g, s, 0) for s in sets])
-        b.use_graph(True)
-        ms = []
-        for _ in range(6):
-            b.run()
-            b.wait()
-            ms.append(b.last_ms())
-        t = float(np.median(ms[1:]))
-    out["beam_4096sets_1e5E_mixed"] = {"kernel_ms": t,
-                                       "evals_per_s": 4096 * 1e5 / (t / 1e3)}
+    pol = {"mixed": "PolicyMixed", "fp32": "PolicyF32", "fp64": "PolicyF64"}
+    for prec in ("mixed", "fp32", "fp64"):
+        with Batch(prec) as b:
+            g = b.add_beam_grid(bev, 2.6, 1300.0)
+            b.set_jobs([make_job(BEAM, g, s, 0) for s in sets])
+            b.use_graph(True)
+            ms = []
+            for _ in range(11):
+                b.run()
+                b.wait()
+                ms.append(b.last_ms())
+            t = float(np.median(ms[1:]))
+        beam = {"kernel_ms": t, "evals_per_s": 4096 * 1e5 / (t / 1e3)}
+        # the beam launch writes 9 probabilities per evaluation and reads
+        # little (rocprofv3 FETCH_SIZE 0.07-0.8 GB against 14.7-29.5 GB
+        # written); its arithmetic (the VALU FLOP counters of this launch)
+        # bounds it before HBM does
+        out_b = 4096 * 1e5 * 9 * (8 if prec == "fp64" else 4)
+        beam["output_TBps"] = out_b / (t / 1e3) / 1e12
+        beam["output_frac_of_hbm"] = out_b / (t / 1e3) / PEAK_HBM
+        rec = FL.measured_flops(f"beam_batch_kernel<{pol[prec]}>")
+        if rec:
+            ach = (rec["f64_flop"] + rec["f32_flop"] * PEAK_F64 / PEAK_F32) / (t / 1e3)
+            beam["valu_achieved_TFLOPs"] = ach / 1e12
+            beam["valu_frac_of_fp64_peak"] = ach / PEAK_F64
+        out["beam_4096sets_1e5E_" + prec] = beam
     return out
 
 
