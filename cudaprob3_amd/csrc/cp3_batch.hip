@@ -70,7 +70,9 @@ struct cp3_batch {
 
   int tune_nc = 0, tune_w = 0, tune_waves = 0;  // 0 = automatic
   int tune_mode = CP3_SOLVE_AUTO;
-  int nc = 1, w = 1, mode = CP3_SOLVE_SHARED, a_grid = 0;
+  // nc, w: cosine block and group waves of the propagation; tnc: the packed
+  // tails' cosine block
+  int nc = 1, w = 1, tnc = 1, mode = CP3_SOLVE_SHARED, a_grid = 0;
   int64_t b_max_units = 0;  // units of the longest beam job
 
   bool use_graph = false;
@@ -292,6 +294,11 @@ struct cp3_batch {
     nc = tune_nc ? tune_nc : (mode == CP3_SOLVE_TABLE ? 4 : auto_nc());
     // the table replaces the waves' shared solves: one wave per workgroup
     w = mode == CP3_SOLVE_TABLE ? 1 : (tune_w ? tune_w : 4);
+    // Table-mode tails take at most 4 cosines per lane: on the 1024-set
+    // 59-shell batch (MI355X) the tail kernel takes 0.345 ms at block 4 and
+    // 0.40 ms at blocks 8 and 2, while the propagation kernel is fastest at
+    // block 8 or 4 (6.78 and 6.92 ms).
+    tnc = mode == CP3_SOLVE_TABLE ? std::min(nc, 4) : nc;
     int64_t u = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
       if (jobs[j].kind != CP3_JOB_ATMOS) continue;
@@ -303,7 +310,7 @@ struct cp3_batch {
     u = 0;
     for (cp3::TailGroup& t : tgroups) {
       t.unit_begin = u;
-      u += cp3::tail_units(t, nc, w);
+      u += cp3::tail_units(t, tnc, w);
     }
     t_units = u;
     u = 0;
@@ -373,7 +380,7 @@ struct cp3_batch {
     if (t_units > 0) {
       const cp3::TailGroup* tg = (const cp3::TailGroup*)d_tgroups.p;
       const int ng = (int)tgroups.size();
-      hip_check(table ? cp3::launch_tail_table(precision, nc, aj, tg, ng, t_units, stream)
+      hip_check(table ? cp3::launch_tail_table(precision, tnc, aj, tg, ng, t_units, stream)
                       : cp3::launch_atmos_tail(precision, nc, w, aj, tg, ng, t_units, stream),
                 "packed tail launch");
     }

@@ -70,6 +70,18 @@ __device__ __forceinline__ void apply_shell(Traj<P> (&t)[NC], const Slab<P>& s,
   }
 }
 
+// Shell i as a middle shell of the trajectories with more than i + 1 shells
+// (the others have reached their innermost shell, taken later).
+template <class P, int NC>
+__device__ __forceinline__ void middle_some(Traj<P> (&t)[NC], const Slab<P>& s,
+                                            const GridDev& g, const int (&cidx)[NC],
+                                            const int (&Mc)[NC], int i) {
+  using D = typename P::D;
+#pragma unroll
+  for (int j = 0; j < NC; ++j)
+    if (i < Mc[j] - 1) traj_middle<P>(t[j], s, (D)uload(g.seg, (int64_t)cidx[j] * g.S + i));
+}
+
 // LDS staging of one chunk of W slabs, two buffers; struct of arrays with one
 // entry per lane so every access is a conflict-free contiguous row.
 template <class P, int W>
@@ -170,35 +182,43 @@ struct DevBins {
   __device__ O weight(int a, int bin) const { return w[((int64_t)a * n + bin) * plane]; }
 };
 
-// The atmosphere segment and the nine probabilities of the NC trajectories,
-// stored as rows of the [9][nC][nE] output (AVG: averaged over the
-// production-height bins of pha).
+// The atmosphere segment and the nine probabilities of trajectory j of a
+// cosine set, stored as rows of the [9][nC][nE] output (AVG: averaged over
+// the production-height bins of pha).
 template <class P, int NC, bool AVG = false>
-__device__ __forceinline__ void finish_store(const Traj<P> (&t)[NC], const Slab<P>& vac,
-                                             const GridDev& g, const CosSet<NC>& cs,
-                                             const SetConst& sc, void* outp, int e,
-                                             bool e_ok, const PhaDev& pha) {
+__device__ __forceinline__ void finish_store_one(const Traj<P>& t, int j, const Slab<P>& vac,
+                                                 const GridDev& g, const CosSet<NC>& cs,
+                                                 const SetConst& sc, void* outp, int e,
+                                                 bool e_ok, const PhaDev& pha) {
   using D = typename P::D;
   using F = typename P::F;
   using O = typename OutT<P>::type;
   O* out = (O*)outp;
   const int64_t plane = (int64_t)g.nC * g.nE;
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    F prob[9];
-    if constexpr (AVG) {
-      const DevBins<P> bins(pha, g, cs.cidx[j], e_ok ? e : g.nE - 1);
-      traj_finish_avg<P>(t[j], vac, (D)uload(g.Lvac, cs.cidx[j]), sc, bins, prob);
-    } else {
-      (void)pha;
-      traj_finish<P>(t[j], vac, (D)uload(g.Lvac, cs.cidx[j]), sc, prob);
-    }
-    if (e_ok && cs.ok[j]) {
-      O* dst = out + (int64_t)cs.cidx[j] * g.nE + e;
-#pragma unroll
-      for (int ch = 0; ch < 9; ++ch) dst[ch * plane] = (O)prob[ch];
-    }
+  F prob[9];
+  if constexpr (AVG) {
+    const DevBins<P> bins(pha, g, cs.cidx[j], e_ok ? e : g.nE - 1);
+    traj_finish_avg<P>(t, vac, (D)uload(g.Lvac, cs.cidx[j]), sc, bins, prob);
+  } else {
+    (void)pha;
+    traj_finish<P>(t, vac, (D)uload(g.Lvac, cs.cidx[j]), sc, prob);
   }
+  if (e_ok && cs.ok[j]) {
+    O* dst = out + (int64_t)cs.cidx[j] * g.nE + e;
+#pragma unroll
+    for (int ch = 0; ch < 9; ++ch) dst[ch * plane] = (O)prob[ch];
+  }
+}
+
+// The same for all NC trajectories.
+template <class P, int NC, bool AVG = false>
+__device__ __forceinline__ void finish_store(const Traj<P> (&t)[NC], const Slab<P>& vac,
+                                             const GridDev& g, const CosSet<NC>& cs,
+                                             const SetConst& sc, void* outp, int e,
+                                             bool e_ok, const PhaDev& pha) {
+#pragma unroll
+  for (int j = 0; j < NC; ++j)
+    finish_store_one<P, NC, AVG>(t[j], j, vac, g, cs, sc, outp, e, e_ok, pha);
 }
 
 // The trajectories of cosine block sb of grid g for one energy per lane:
@@ -489,13 +509,25 @@ __device__ __forceinline__ void table_block(const GridDev& g, int sb, int e, boo
     middle_all<P, NC>(t, s, g, cs.cidx, i);
     s = nx;
   }
-  for (; i < cs.Mmax; ++i) {
+  for (; i < cs.Mmax - 1; ++i) {
     const Slab<P> nx = col[(int64_t)(i + 1) * row];
-    apply_shell<P, NC>(t, s, g, cs.cidx, cs.Mc, i);
+    middle_some<P, NC>(t, s, g, cs.cidx, cs.Mc, i);
     s = nx;
   }
+  // then each trajectory's innermost shell (its own row) and its finish, one
+  // trajectory at a time: the state of the finished ones is dead, and the
+  // shell loop above never holds the innermost shell's temporaries
   const Slab<P> vac = col[(int64_t)g.S * row];
-  finish_store<P, NC, AVG>(t, vac, g, cs, sc, outp, e, e_ok, pha);
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    if (cs.Mc[j] >= 1) {
+      using D = typename P::D;
+      const int im = cs.Mc[j] - 1;
+      const Slab<P> si = col[(int64_t)im * row];
+      traj_inner<P>(t[j], si, (D)uload(g.seg, (int64_t)cs.cidx[j] * g.S + im));
+    }
+    finish_store_one<P, NC, AVG>(t[j], j, vac, g, cs, sc, outp, e, e_ok, pha);
+  }
 }
 
 // A work unit of one job in table mode: energy tile x cosine block, the
