@@ -356,16 +356,16 @@ def main():
     traffic_rec = FL.measured_traffic(dom)
     traffic = traffic_rec["bytes"] if traffic_rec else None
     # the issue view: the kernel's VALU instructions (rocprofv3 SQ_INSTS_VALU,
-    # same tuning and workload) over its launch time in this run, against the
-    # rate an FMA-chain probe reaches at the kernel's occupancy
+    # same kernel and workload) over its launch time in this run, against the
+    # rate its instruction mix allows at its occupancy (probe rates per form)
     valu_rec = FL.measured_valu(dom)
     valu = None
     if valu_rec:
-        rate = valu_rec["sq_insts_valu_per_launch"] / t_dom / (N_CU * CLOCK_HZ)
-        cap = valu_rec["probe_fp32_fma_wave_instr_per_cu_cycle"]["4_waves_per_simd"]
-        valu = {"wave_instr_per_cu_cycle": rate, "probe_at_same_occupancy": cap,
-                "frac": rate / cap, "clock_hz_assumed": CLOCK_HZ,
-                "detail": valu_rec}
+        rate = valu_rec["counters"]["VALU"] / t_dom / (N_CU * CLOCK_HZ)
+        ceil = FL.issue_ceiling(valu_rec)
+        valu = {"wave_instr_per_cu_cycle": rate, "mix_ceiling": ceil["ceiling"],
+                "frac": rate / ceil["ceiling"], "clock_hz_assumed": CLOCK_HZ,
+                "detail": ceil}
 
     # the same rate from rocprofv3's VALU FLOP counters of this kernel on this
     # workload (instruction counts, committed), over this run's launch time
@@ -465,6 +465,10 @@ def main():
             "kernel": kernel_name("fp64", tune_d),
             "frac_of_fp64_peak": c64["f64"] / t64 / PEAK_F64,
         }
+        rec64 = FL.measured_flops(kernel_name("fp64", tune_d))
+        if rec64:
+            result["fp64"]["frac_of_fp64_peak_counters"] = (
+                (rec64["f64_flop"] + rec64["f32_flop"] * PEAK_F64 / PEAK_F32) / t64 / PEAK_F64)
         result["mixed_over_fp64"] = value / v64
         result["mixed_err_vs_fp64_gpu"] = {"max": worst,
                                            "rms": float(np.sqrt(sq / n))}

@@ -149,15 +149,40 @@ def beam_counts(n_points: int, precision: str) -> dict:
 
 
 def measured_valu(name: str):
-    """VALU wave-instructions per launch measured with rocprofv3 counters
-    (SQ_INSTS_VALU) and the issue rates of an FMA-chain probe, as recorded in
-    profiles/valu_counters.json, or None."""
+    """The VALU issue record of a kernel, or None: its occupancy (compiler
+    resource usage) and the per-form issue rates of a probe of independent
+    instruction chains, from profiles/valu_counters.json, with the kernel's
+    instruction counts per launch (rocprofv3 SQ_INSTS_VALU and the per-type
+    counters) from profiles/valu_flops.json."""
     path = os.path.join(PROFILES, "valu_counters.json")
-    if not os.path.exists(path):
+    flops = measured_flops(name)
+    if not os.path.exists(path) or flops is None:
         return None
     with open(path) as f:
         data = json.load(f)
-    return data.get(name)
+    occ = data["kernels"].get(name)
+    if occ is None:
+        return None
+    return {"occupancy": occ, "probe": data["probe"],
+            "counters": flops["counters_per_launch"]}
+
+
+def issue_ceiling(rec) -> dict:
+    """Wave-instructions per CU-cycle the kernel's instruction mix allows at
+    its occupancy, from the probe's per-form rates: FP32 arithmetic at the
+    rate of two-operand FP32 forms with VGPR operands, FP64 arithmetic and
+    conversions at the FP64 rate, everything else at the FP32 rate."""
+    w = str(rec["occupancy"]["waves_per_simd"])
+    p = rec["probe"]
+    r32 = p["fp32_vop2_vgpr_operands"].get(w) or p["fp32_vop2_vgpr_operands"]["4"]
+    r64 = p["fp64_fmac"].get(w) or p["fp64_fmac"]["4"]
+    c = rec["counters"]
+    f32 = c["ADD_F32"] + c["MUL_F32"] + c["FMA_F32"]
+    f64 = c["ADD_F64"] + c["MUL_F64"] + c["FMA_F64"] + c["TRANS_F64"] + c["CVT"]
+    rest = max(c["VALU"] - f32 - f64, 0.0)
+    cycles = (f32 + rest) / r32 + f64 / r64
+    return {"ceiling": c["VALU"] / cycles, "fp32_rate": r32, "fp64_rate": r64,
+            "waves_per_simd": rec["occupancy"]["waves_per_simd"]}
 
 
 def measured_traffic(name: str):
