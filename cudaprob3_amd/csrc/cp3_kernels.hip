@@ -432,10 +432,61 @@ __global__ void __launch_bounds__(kWave * W, (MinWaves<P, NC>::value)) atmos_tai
 //
 // The batch engine can solve every (job, shell, energy) slab once, in a
 // launch of its own, into a table in HBM; the propagation kernels then read
-// each slab with one coalesced row load per wave instead of solving it in
-// every work unit (a work unit shares a solve among its own trajectories, the
-// table among every cosine of the grid).  Table of a job: Slab<P> rows
-// [g.S + 1][energy_tiles(g) * 64], row g.S the vacuum.
+// each slab with coalesced row loads instead of solving it in every work
+// unit (a work unit shares a solve among its own trajectories, the table
+// among every cosine of the grid).  Table of a job: Slab<P> rows
+// [g.S + 1][energy_tiles(g) * 64], row g.S the vacuum, stored by tile as
+// TileIO lays it out.
+
+// The 64 entries of one (row, energy tile) of a slab table, word-major: the
+// first 16 bytes of every lane's entry, then the next 16 bytes, and a last
+// part of 8 bytes likewise, so that each load or store instruction of a
+// wave touches one contiguous block (1 KiB, or 512 bytes for the last part)
+// instead of every line of the tile's span.
+template <class T>
+struct TileIO {
+  static_assert(sizeof(T) % 8 == 0, "slab entries are whole 8-byte words");
+  static constexpr int kQuads = (int)(sizeof(T) / 16);
+  static constexpr bool kHalf = sizeof(T) % 16 != 0;
+  static constexpr int64_t kTileBytes = (int64_t)sizeof(T) * kWave;
+  __device__ static T load(const char* tile, int lane) {
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (int k = 0; k < kQuads; ++k) {
+      const uint4 v = ((const uint4*)(tile + k * 16 * kWave))[lane];
+      w[4 * k] = v.x;
+      w[4 * k + 1] = v.y;
+      w[4 * k + 2] = v.z;
+      w[4 * k + 3] = v.w;
+    }
+    if constexpr (kHalf) {
+      const uint2 v = ((const uint2*)(tile + kQuads * 16 * kWave))[lane];
+      w[4 * kQuads] = v.x;
+      w[4 * kQuads + 1] = v.y;
+    }
+    T t;
+    __builtin_memcpy(&t, w, sizeof(T));
+    return t;
+  }
+  __device__ static void store(char* tile, int lane, const T& t) {
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &t, sizeof(T));
+#pragma unroll
+    for (int k = 0; k < kQuads; ++k)
+      ((uint4*)(tile + k * 16 * kWave))[lane] =
+          make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    if constexpr (kHalf)
+      ((uint2*)(tile + kQuads * 16 * kWave))[lane] = make_uint2(w[4 * kQuads], w[4 * kQuads + 1]);
+  }
+};
+
+// The tile block of row i and energy e of a job's table (lane e % 64 of it).
+template <class P>
+__device__ __forceinline__ const char* slab_tile(const void* slabs, const GridDev& g, int i,
+                                                 int e) {
+  return (const char*)slabs +
+         ((int64_t)i * energy_tiles(g) + e / kWave) * TileIO<Slab<P>>::kTileBytes;
+}
 
 // Pre-pass: the (S + 1) x nE used entries of each job's table, 64
 // consecutive (row, energy) pairs per wave, row-major, so that no lane solves
@@ -461,7 +512,7 @@ __device__ __forceinline__ void slab_wave(const AtmosJob& job, int64_t w) {
   const D rye = (D)g.rye[i < g.S ? i : 0];
   Slab<P> sl;
   solve_slab<P>(job.sc, E, D(1) / E, i < g.S ? rye : D(0), sl);
-  ((Slab<P>*)job.slabs)[(int64_t)i * energy_tiles(g) * kWave + e] = sl;
+  TileIO<Slab<P>>::store(const_cast<char*>(slab_tile<P>(job.slabs, g, i, e)), e % kWave, sl);
 }
 
 // The batch's pre-pass on a 2-D grid (y: job, x: the job's waves) so that a
@@ -489,41 +540,43 @@ __global__ void __launch_bounds__(kWave) slab_table_single_kernel(AtmosJob job,
 }
 
 // The NC trajectories of cosine block sb (NC cosines) for one energy per
-// lane, reading slab row i of the lane's table column `col` (row stride
-// `row` entries), each row loaded one shell ahead of its use.
+// lane, reading slab row i of the lane's table column: entry `lane` of the
+// tile block col + i * row (row: bytes per table row), each row loaded one
+// shell ahead of its use.
 template <class P, int NC, bool AVG = false>
 __device__ __forceinline__ void table_block(const GridDev& g, int sb, int e, bool e_ok,
-                                            const Slab<P>* col, int64_t row,
+                                            const char* col, int lane, int64_t row,
                                             const SetConst& sc, void* outp,
                                             const PhaDev& pha) {
+  using IO = TileIO<Slab<P>>;
   CosSet<NC> cs;
   cos_set<NC>(g, sb * NC, cs);
   Traj<P> t[NC];
 #pragma unroll
   for (int j = 0; j < NC; ++j) traj_init<P>(t[j], cs.Mc[j] == 0);
   // rows i + 1 <= Mmax <= g.S all exist (row g.S: the vacuum)
-  Slab<P> s = col[0];
+  Slab<P> s = IO::load(col, lane);
   int i = 0;
   for (; i < cs.Mmin - 1; ++i) {
-    const Slab<P> nx = col[(int64_t)(i + 1) * row];
+    const Slab<P> nx = IO::load(col + (i + 1) * row, lane);
     middle_all<P, NC>(t, s, g, cs.cidx, i);
     s = nx;
   }
   for (; i < cs.Mmax - 1; ++i) {
-    const Slab<P> nx = col[(int64_t)(i + 1) * row];
+    const Slab<P> nx = IO::load(col + (i + 1) * row, lane);
     middle_some<P, NC>(t, s, g, cs.cidx, cs.Mc, i);
     s = nx;
   }
   // then each trajectory's innermost shell (its own row) and its finish, one
   // trajectory at a time: the state of the finished ones is dead, and the
   // shell loop above never holds the innermost shell's temporaries
-  const Slab<P> vac = col[(int64_t)g.S * row];
+  const Slab<P> vac = IO::load(col + g.S * row, lane);
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     if (cs.Mc[j] >= 1) {
       using D = typename P::D;
       const int im = cs.Mc[j] - 1;
-      const Slab<P> si = col[(int64_t)im * row];
+      const Slab<P> si = IO::load(col + im * row, lane);
       traj_inner<P>(t[j], si, (D)uload(g.seg, (int64_t)cs.cidx[j] * g.S + im));
     }
     finish_store_one<P, NC, AVG>(t[j], j, vac, g, cs, sc, outp, e, e_ok, pha);
@@ -539,9 +592,9 @@ __device__ __forceinline__ void table_unit(const AtmosJob& job, int64_t local) {
   const int et = (int)(local / ncb);
   const int sb = (int)(local - (int64_t)et * ncb);
   const int e = et * kWave + lane;
-  const int64_t row = (int64_t)energy_tiles(job.g) * kWave;
-  table_block<P, NC, AVG>(job.g, sb, e, e < job.g.nE, (const Slab<P>*)job.slabs + e, row,
-                          job.sc, job.out, job.pha);
+  const int64_t row = energy_tiles(job.g) * TileIO<Slab<P>>::kTileBytes;
+  table_block<P, NC, AVG>(job.g, sb, e, e < job.g.nE, slab_tile<P>(job.slabs, job.g, 0, e),
+                          lane, row, job.sc, job.out, job.pha);
 }
 
 // A packed tail unit in table mode: packed tile x cosine block.
@@ -557,15 +610,22 @@ __device__ __forceinline__ void tail_table_unit(const AtmosJob* jobs, const Tail
   const bool ok = q < tg.per && slot < tg.njobs;
   const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
   const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
-  const int64_t row = (int64_t)energy_tiles(tg.g) * kWave;
-  table_block<P, NC>(tg.g, sb, e, ok, (const Slab<P>*)job.slabs + e, row, job.sc, job.out,
-                     job.pha);
+  const int64_t row = energy_tiles(tg.g) * TileIO<Slab<P>>::kTileBytes;
+  table_block<P, NC>(tg.g, sb, e, ok, slab_tile<P>(job.slabs, tg.g, 0, e), e % kWave, row,
+                     job.sc, job.out, job.pha);
 }
 
 // Minimum waves per SIMD of the table-mode kernels (0: natural allocation).
+// The mixed kernel with 8 cosines per lane is held to 3 waves (168 VGPRs,
+// a few spilled): left alone the scheduler takes 235 VGPRs and 2 waves,
+// 13% slower on the 1024-set, 59-shell batch on MI355X.
 template <class P, int NC>
 struct MinWavesTable {
   static constexpr int value = 0;
+};
+template <int NC>
+struct MinWavesTable<PolicyMixed, NC> {
+  static constexpr int value = NC == 8 ? 3 : 0;
 };
 
 
