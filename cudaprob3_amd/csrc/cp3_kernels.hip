@@ -39,6 +39,23 @@ __device__ __forceinline__ T uload(const T* p, int64_t i) {
   return ((const __attribute__((address_space(4))) T*)p)[i];
 }
 
+// Per-lane accesses through the global address space.  Pointers taken from
+// job descriptors are generic, and a generic (flat) access counts against the
+// scalar-memory/LDS wait counter as well as the vector-memory one: every wait
+// for a scalar load (a segment length, a shell count) would also wait for the
+// slab-row prefetch and the output stores in flight.  A global access counts
+// against the vector-memory counter only.  No pointer so accessed points into
+// LDS or private memory (they are device buffers, the static scratch array or
+// a caller's device-accessible output), which is all a global access needs.
+template <class T>
+__device__ __forceinline__ T gload(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+template <class T>
+__device__ __forceinline__ void gstore(T* p, int64_t i, T v) {
+  ((__attribute__((address_space(1))) T*)p)[i] = v;
+}
+
 // Shell i as a middle shell of all NC trajectories of a wave (i < Mmin - 1,
 // Mmin the smallest shell count among them): branch-free, so the NC
 // independent update chains interleave.
@@ -179,7 +196,7 @@ struct DevBins {
     hm = (D)uload(hmw, 2 * bin);
     hw = (D)uload(hmw, 2 * bin + 1);
   }
-  __device__ O weight(int a, int bin) const { return w[((int64_t)a * n + bin) * plane]; }
+  __device__ O weight(int a, int bin) const { return gload(w, ((int64_t)a * n + bin) * plane); }
 };
 
 // The atmosphere segment and the nine probabilities of trajectory j of a
@@ -206,7 +223,7 @@ __device__ __forceinline__ void finish_store_one(const Traj<P>& t, int j, const 
   if (e_ok && cs.ok[j]) {
     O* dst = out + (int64_t)cs.cidx[j] * g.nE + e;
 #pragma unroll
-    for (int ch = 0; ch < 9; ++ch) dst[ch * plane] = (O)prob[ch];
+    for (int ch = 0; ch < 9; ++ch) gstore(dst, ch * plane, (O)prob[ch]);
   }
 }
 
@@ -232,7 +249,7 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
   using D = typename P::D;
   const int lane = (int)threadIdx.x % kWave;
   const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-  const D E = (D)g.ev[e_ok ? e : g.nE - 1];
+  const D E = (D)gload(g.ev, e_ok ? e : g.nE - 1);
   const D inv = D(1) / E;
 
   const int base = sb * (W * NC);
@@ -337,7 +354,7 @@ __device__ __forceinline__ void atmos_tail_unit(const AtmosJob* jobs,
   const int slot = t * tg.per + q;
   const bool ok = q < tg.per && slot < tg.njobs;
   const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
-  const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
+  const AtmosJob& job = jobs[gload(tg.jobs, ok ? slot : 0)];
   atmos_block<P, NC, W>(tg.g, sb, e, ok, job.sc, job.out, job.pha, st);
 }
 
@@ -449,18 +466,22 @@ struct TileIO {
   static constexpr int kQuads = (int)(sizeof(T) / 16);
   static constexpr bool kHalf = sizeof(T) % 16 != 0;
   static constexpr int64_t kTileBytes = (int64_t)sizeof(T) * kWave;
+  // clang vector types (not HIP's vector classes) so that the accesses can
+  // go through gload / gstore
+  typedef uint32_t W4 __attribute__((ext_vector_type(4)));
+  typedef uint32_t W2 __attribute__((ext_vector_type(2)));
   __device__ static T load(const char* tile, int lane) {
     uint32_t w[sizeof(T) / 4];
 #pragma unroll
     for (int k = 0; k < kQuads; ++k) {
-      const uint4 v = ((const uint4*)(tile + k * 16 * kWave))[lane];
+      const W4 v = gload((const W4*)(tile + k * 16 * kWave), lane);
       w[4 * k] = v.x;
       w[4 * k + 1] = v.y;
       w[4 * k + 2] = v.z;
       w[4 * k + 3] = v.w;
     }
     if constexpr (kHalf) {
-      const uint2 v = ((const uint2*)(tile + kQuads * 16 * kWave))[lane];
+      const W2 v = gload((const W2*)(tile + kQuads * 16 * kWave), lane);
       w[4 * kQuads] = v.x;
       w[4 * kQuads + 1] = v.y;
     }
@@ -472,11 +493,14 @@ struct TileIO {
     uint32_t w[sizeof(T) / 4];
     __builtin_memcpy(w, &t, sizeof(T));
 #pragma unroll
-    for (int k = 0; k < kQuads; ++k)
-      ((uint4*)(tile + k * 16 * kWave))[lane] =
-          make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
-    if constexpr (kHalf)
-      ((uint2*)(tile + kQuads * 16 * kWave))[lane] = make_uint2(w[4 * kQuads], w[4 * kQuads + 1]);
+    for (int k = 0; k < kQuads; ++k) {
+      const W4 v = {w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
+      gstore((W4*)(tile + k * 16 * kWave), lane, v);
+    }
+    if constexpr (kHalf) {
+      const W2 v = {w[4 * kQuads], w[4 * kQuads + 1]};
+      gstore((W2*)(tile + kQuads * 16 * kWave), lane, v);
+    }
   }
 };
 
@@ -507,9 +531,9 @@ __device__ __forceinline__ void slab_wave(const AtmosJob& job, int64_t w) {
     ++i;
   }
   if (i > g.S) return;  // past the last row
-  const D E = (D)g.ev[e];
+  const D E = (D)gload(g.ev, e);
   // g.S >= 1 (an Earth model has rows), so row 0 of rye exists
-  const D rye = (D)g.rye[i < g.S ? i : 0];
+  const D rye = (D)gload(g.rye, i < g.S ? i : 0);
   Slab<P> sl;
   solve_slab<P>(job.sc, E, D(1) / E, i < g.S ? rye : D(0), sl);
   TileIO<Slab<P>>::store(const_cast<char*>(slab_tile<P>(job.slabs, g, i, e)), e % kWave, sl);
@@ -609,7 +633,7 @@ __device__ __forceinline__ void tail_table_unit(const AtmosJob* jobs, const Tail
   const int slot = t * tg.per + q;
   const bool ok = q < tg.per && slot < tg.njobs;
   const int e = tg.g.nE - tg.tail + (lane - q * tg.tail);
-  const AtmosJob& job = jobs[tg.jobs[ok ? slot : 0]];
+  const AtmosJob& job = jobs[gload(tg.jobs, ok ? slot : 0)];
   const int64_t row = energy_tiles(tg.g) * TileIO<Slab<P>>::kTileBytes;
   table_block<P, NC>(tg.g, sb, e, ok, slab_tile<P>(job.slabs, tg.g, 0, e), e % kWave, row,
                      job.sc, job.out, job.pha);
@@ -695,13 +719,13 @@ __device__ __forceinline__ void beam_unit(const BeamJob& job, int64_t local, int
   using O = typename OutT<P>::type;
   const int e = (int)local * kWave + lane;
   const bool e_ok = e < job.nE;
-  const D E = (D)job.ev[e_ok ? e : job.nE - 1];
+  const D E = (D)gload(job.ev, e_ok ? e : job.nE - 1);
   F prob[9];
   beam_point<P>(job.sc, E, (D)job.rye, (D)job.L, prob);
   if (e_ok) {
     O* out = (O*)job.out;
 #pragma unroll
-    for (int ch = 0; ch < 9; ++ch) out[(int64_t)ch * job.nE + e] = (O)prob[ch];
+    for (int ch = 0; ch < 9; ++ch) gstore(out, (int64_t)ch * job.nE + e, (O)prob[ch]);
   }
 }
 
@@ -740,14 +764,14 @@ __global__ void __launch_bounds__(kWave * kBeamWaves) beam_batch_kernel(
 #pragma unroll
   for (int k = 0; k < kBeamPerLane; ++k) {
     e[k] = (int)local * kBeamPerLane * kWave + k * kWave + lane;
-    const D E = (D)job.ev[e[k] < job.nE ? e[k] : job.nE - 1];
+    const D E = (D)gload(job.ev, e[k] < job.nE ? e[k] : job.nE - 1);
     beam_point<P>(job.sc, E, (D)job.rye, (D)job.L, p[k]);
   }
 #pragma unroll
   for (int k = 0; k < kBeamPerLane; ++k)
     if (e[k] < job.nE) {
 #pragma unroll
-      for (int ch = 0; ch < 9; ++ch) out[(int64_t)ch * job.nE + e[k]] = (O)p[k][ch];
+      for (int ch = 0; ch < 9; ++ch) gstore(out, (int64_t)ch * job.nE + e[k], (O)p[k][ch]);
     }
 }
 
