@@ -320,6 +320,52 @@ CP3_HD void eigvec(const D H[6], const D H2[6], D lk, D lp, D lq, F v[3]) {
   }
 }
 
+// cos and sin of the cubic's third angle acos(x) / 3 (physics.hpp:310).
+// FP32: acos and sincos.  FP64: e^{i acos(x) / 3} is the principal cube root
+// of z = x + i sqrt(1 - x^2), found by Newton's method for u^3 = z,
+//   u <- u + (z / u^2 - u) / 3,  z / u^2 = z conj(u)^2 / |u|^4,
+// from a seed of FP32 accuracy.  1 / |u|^4 is taken as (2 - |u|^2)^2, which
+// differs from it by O((|u|^2 - 1)^2) and keeps the convergence quadratic
+// (at u = u* (1 + e) the step leaves u* + O(e^2)), so two steps reach FP64
+// rounding: about 50 FP64 operations instead of an FP64 acos and sincos
+// (about 145 instructions on gfx950, acos with a divergent branch).
+// 1 - x^2 is formed as (1 - x)(1 + x), accurate to a few ulp up to |x| = 1.
+CP3_HD void m_third_angle_seed(double x, double* c, double* s) {
+  const float t = acosf((float)x) * (1.0f / 3.0f);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float r = t * 0.15915494309189535f;  // revolutions
+  *c = (double)__builtin_amdgcn_cosf(r);
+  *s = (double)__builtin_amdgcn_sinf(r);
+#else
+  *c = (double)cosf(t);
+  *s = (double)sinf(t);
+#endif
+}
+
+CP3_HD void third_angle(float x, float* c, float* s) {
+  m_sincos(m_acos(x) * (1.0f / 3.0f), s, c);
+}
+
+template <class D>
+CP3_HD void third_angle(D x, D* c, D* s) {
+  const D w = m_sqrt((D(1) - x) * (D(1) + x));
+  D uc, us;
+  m_third_angle_seed(x, &uc, &us);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const D c2 = uc * uc, s2 = us * us, cs = uc * us;
+    const D t = D(2) - (c2 + s2);
+    const D q = t * t;
+    const D a = c2 - s2;  // conj(u)^2 = a - 2i cs
+    const D pr = x * a + D(2) * (w * cs);
+    const D pi = w * a - D(2) * (x * cs);
+    uc = uc + (pr * q - uc) * D(1.0 / 3.0);
+    us = us + (pi * q - us) * D(1.0 / 3.0);
+  }
+  *c = uc;
+  *s = us;
+}
+
 // Matter eigenvalues in the reference's order: the trigonometric roots of
 // the cubic with physics.hpp:283-323's coefficients, in ORDER's sequence
 // through the per-set root coefficients ra, rb.
@@ -336,9 +382,8 @@ CP3_HD void matter_eigs(const SetConst& sc, D fac, D lam[3]) {
            D(27) * gamma) /
           (D(2) * (tmp * st));
   if (m_fabs(arg) > D(1)) arg = arg > D(0) ? D(1) : D(-1);
-  const D th0 = m_acos(arg) * D(1.0 / 3.0);
   D sn, cs;
-  m_sincos(th0, &sn, &cs);
+  third_angle(arg, &cs, &sn);
   const D amp = D(-2.0 / 3.0) * st;
   const D shift = D(sc.dm00) - alpha * D(1.0 / 3.0);
 #pragma unroll

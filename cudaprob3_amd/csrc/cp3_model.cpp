@@ -203,6 +203,18 @@ inline void m_sincos(Cnt<float> x, Cnt<float>* s, Cnt<float>* c) {
   s->v = std::sin(x.v);
   c->v = std::cos(x.v);
 }
+// The FP64 third angle's FP32 seed: an FP32 acos, one FP32 multiply and an
+// FP32 sincos; the Newton steps after it are counted as they run.
+inline void m_third_angle_seed(Cnt<double> x, Cnt<double>* c, Cnt<double>* s) {
+  g_ops.acos32 += 1;
+  g_ops.sincos32 += 1;
+  g_ops.f32 += 1;
+  cp3::m_third_angle_seed(x.v, &c->v, &s->v);
+}
+// The FP32 form's third angle: acos and sincos, counted by their overloads.
+inline void third_angle(Cnt<float> x, Cnt<float>* c, Cnt<float>* s) {
+  m_sincos(m_acos(x) * Cnt<float>(1.0 / 3.0), s, c);
+}
 
 }  // namespace
 

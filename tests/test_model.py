@@ -123,8 +123,12 @@ def test_stage_counts():
     assert mixed["middle"]["f64"] == 2 * 2 + 3
     assert f64["middle"]["f64"] == 4 * 28 + 2 * 3
     assert f64["middle"]["sincos64"] == 2 and mixed["middle"]["sincos32"] == 2
-    assert f64["solve"]["acos64"] == 1 and mixed["solve"]["acos64"] == 1
-    assert mixed["solve"]["f32"] == 0  # the eigen-solve stays FP64
+    # the cubic's third angle: an FP32 acos and sincos seed (one FP32
+    # multiply), then FP64 Newton steps for the cube root of e^{i acos(x)}
+    for form in (f64, mixed):
+        assert form["solve"]["acos64"] == 0 and form["solve"]["sincos64"] == 0
+        assert form["solve"]["acos32"] == 1 and form["solve"]["sincos32"] == 1
+    assert mixed["solve"]["f32"] == 1  # the eigen-solve's arithmetic stays FP64
     # ... but for the beam's F eigenvectors, normalised in F: its FP64 work is
     # below one solve plus its two phase reductions (q = om L, q - rint(q))
     assert mixed["beam"]["f64"] < mixed["solve"]["f64"] + 2 * 2
@@ -132,8 +136,10 @@ def test_stage_counts():
     wl = W.atmos1024(n_sets=4)
     c1 = FL.batch_counts(wl, "mixed", 1)
     c8 = FL.batch_counts(wl, "mixed", 8)
-    assert c8["acos64"] < c1["acos64"] / 6
-    assert c8["f32"] == c1["f32"]
+    assert c8["acos32"] < c1["acos32"] / 6
+    # the layer arithmetic does not depend on the blocking (one FP32 seed
+    # multiply per solve aside)
+    assert c8["f32"] - c8["acos32"] == c1["f32"] - c1["acos32"]
 
 
 def test_phase_polynomial():
