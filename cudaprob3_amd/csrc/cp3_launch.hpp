@@ -119,9 +119,10 @@ hipError_t launch_atmos_batch(int precision, int nc, int w, const AtmosJob* d_jo
                               int njobs, int64_t nunits, int grid,
                               hipStream_t stream);
 // Slab-table mode: bytes per table entry (sizeof(Slab<P>)), the pre-pass
-// over every job's table (a max_waves x njobs grid, max_waves the largest
-// slab_waves; at most 65535 jobs), and the propagation launches (one wave
-// per workgroup) reading the tables.
+// over every job's table (max_waves 64-entry tiles per job, max_waves the
+// largest slab_waves, a few tiles per one-wave workgroup; at most 65535
+// jobs), and the propagation launches (one wave per workgroup) reading the
+// tables.
 size_t slab_entry_bytes(int precision);
 // The single grid's pre-pass (job by value), which also stores the job
 // descriptor at d_job for the propagation launch that follows.
