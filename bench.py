@@ -366,6 +366,17 @@ def main():
         valu = {"wave_instr_per_cu_cycle": rate, "mix_ceiling": ceil["ceiling"],
                 "frac": rate / ceil["ceiling"], "clock_hz_assumed": CLOCK_HZ,
                 "detail": ceil}
+        # the same per cycle of the clock the kernel runs at under load
+        # (SQ_BUSY_CYCLES of its profile: below the nominal 2.4 GHz), against
+        # the probe's rates per cycle of its own measured clock
+        clk = valu_rec["occupancy"].get("clock_hz_under_load")
+        if clk and valu_rec.get("probe_per_clock"):
+            rate_c = valu_rec["counters"]["VALU"] / t_dom / (N_CU * clk)
+            ceil_c = FL.issue_ceiling(valu_rec, per_clock=True)
+            valu["at_clock_under_load"] = {
+                "clock_hz": clk, "wave_instr_per_cu_cycle": rate_c,
+                "mix_ceiling": ceil_c["ceiling"], "frac": rate_c / ceil_c["ceiling"],
+                "source": "profiles/atmos_table_kernel_valu_issue_59shell_1024sets.md"}
 
     # the same rate from rocprofv3's VALU FLOP counters of this kernel on this
     # workload (instruction counts, committed), over this run's launch time
@@ -420,6 +431,15 @@ def main():
             "achieved_from": "rocprofv3 VALU FLOP counters" if counters else "counted FLOPs",
             "achieved_counted": achieved / 1e12,
             "frac_counted": achieved / PEAK_F64,
+            # the published peak assumes the 2.4 GHz peak engine clock; the
+            # same peak scaled to the clock this kernel holds under load
+            "peak_at_clock_under_load": (
+                PEAK_F64 / 1e12 * valu["at_clock_under_load"]["clock_hz"] / CLOCK_HZ
+                if valu and "at_clock_under_load" in valu else None),
+            "frac_at_clock_under_load": (
+                (counters["achieved"] * 1e12 if counters else achieved) /
+                (PEAK_F64 * valu["at_clock_under_load"]["clock_hz"] / CLOCK_HZ)
+                if valu and "at_clock_under_load" in valu else None),
             "traffic": traffic,
             "traffic_detail": traffic_rec,
             "valu_issue": valu,

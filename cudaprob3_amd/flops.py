@@ -164,18 +164,33 @@ def measured_valu(name: str):
     if occ is None:
         return None
     return {"occupancy": occ, "probe": data["probe"],
+            "probe_per_clock": data.get("probe_per_clock"),
             "counters": flops["counters_per_launch"]}
 
 
-def issue_ceiling(rec) -> dict:
+def _rate_at(table: dict, waves: int) -> float:
+    """The probe rate measured at the largest wave count not above `waves`."""
+    keys = sorted(int(k) for k in table)
+    return table[str(max([k for k in keys if k <= waves] or keys[:1]))]
+
+
+def issue_ceiling(rec, per_clock: bool = False) -> dict:
     """Wave-instructions per CU-cycle the kernel's instruction mix allows at
     its occupancy, from the probe's per-form rates: FP32 arithmetic at the
     rate of two-operand FP32 forms with VGPR operands, FP64 arithmetic and
-    conversions at the FP64 rate, everything else at the FP32 rate."""
-    w = str(rec["occupancy"]["waves_per_simd"])
-    p = rec["probe"]
-    r32 = p["fp32_vop2_vgpr_operands"].get(w) or p["fp32_vop2_vgpr_operands"]["4"]
-    r64 = p["fp64_fmac"].get(w) or p["fp64_fmac"]["4"]
+    conversions at the FP64 rate, everything else at the FP32 rate.
+    per_clock: the probe's rates per cycle of the clock it ran at (measured
+    with SQ_BUSY_CYCLES), for a kernel rate taken at its own measured clock;
+    otherwise per cycle of the nominal 2.4 GHz."""
+    w = rec["occupancy"]["waves_per_simd"]
+    if per_clock:
+        p = rec["probe_per_clock"]
+        r32 = _rate_at(p["fp32_fmac_16_chains"], w)
+        r64 = _rate_at(p["fp64_fmac_8_chains"], w)
+    else:
+        p = rec["probe"]
+        r32 = p["fp32_vop2_vgpr_operands"].get(str(w)) or p["fp32_vop2_vgpr_operands"]["4"]
+        r64 = p["fp64_fmac"].get(str(w)) or p["fp64_fmac"]["4"]
     c = rec["counters"]
     f32 = c["ADD_F32"] + c["MUL_F32"] + c["FMA_F32"]
     f64 = c["ADD_F64"] + c["MUL_F64"] + c["FMA_F64"] + c["TRANS_F64"] + c["CVT"]
