@@ -308,6 +308,22 @@ void cp3m_cispi_m1(int precision, const double* f, int n, double* wr,
   }
 }
 
+// cos and sin of the cubic's third angle acos(x) / 3 (third_angle) at n
+// points, in FP64 (precision 0: FP32 seed and Newton steps for the cube root)
+// or FP32 (otherwise: acos and sincos).
+void cp3m_third_angle(int precision, const double* x, int n, double* c, double* s) {
+  for (int k = 0; k < n; ++k) {
+    if (precision == 0) {
+      cp3::third_angle(x[k], &c[k], &s[k]);
+    } else {
+      float cf, sf;
+      cp3::third_angle((float)x[k], &cf, &sf);
+      c[k] = cf;
+      s[k] = sf;
+    }
+  }
+}
+
 // A density file as the product loads it (setDensityFromFile): rows stored
 // descending, 5 values per row (radius, rho or a, b, c, Ye; b = c = 0 for a
 // constant model); *poly = 1 for a polynomial model.  Returns -1 with the

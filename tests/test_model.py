@@ -142,6 +142,41 @@ def test_stage_counts():
     assert c8["f32"] - c8["acos32"] == c1["f32"] - c1["acos32"]
 
 
+def test_third_angle():
+    """cos and sin of acos(x)/3 (physics.hpp:310), which the FP64 forms take
+    as the cube root of x + i sqrt(1 - x^2) by Newton's method from an FP32
+    seed: against 40-digit values over [-1, 1], including both ends and
+    points within 2^-52 of them, where acos is ill-conditioned; FP32 (acos and
+    sincos) within its own rounding."""
+    import mpmath
+    from cudaprob3_amd._native import MODEL_SO
+    m = C.CDLL(MODEL_SO)
+    m.cp3m_third_angle.argtypes = [C.c_int, dp, C.c_int, dp, dp]
+    near = np.array([2.0 ** -k for k in range(1, 53)])
+    x = np.concatenate([np.linspace(-1.0, 1.0, 4001), [0.0, -1.0, 1.0],
+                        1.0 - near, -1.0 + near])
+    mpmath.mp.dps = 40
+
+    def ref(xs):
+        th = [mpmath.acos(mpmath.mpf(float(v))) / 3 for v in xs]
+        return (np.array([float(mpmath.cos(t)) for t in th]),
+                np.array([float(mpmath.sin(t)) for t in th]))
+
+    # the FP32 form sees x rounded to float, so its reference is taken there
+    for prec, xr, tol in ((0, x, 4e-16), (1, x.astype(np.float32).astype(np.float64), 2e-6)):
+        c_ref, s_ref = ref(xr)
+        c, s = np.zeros_like(x), np.zeros_like(x)
+        m.cp3m_third_angle(prec, P(x), len(x), c.ctypes.data_as(dp), s.ctypes.data_as(dp))
+        assert np.abs(c - c_ref).max() < tol, prec
+        assert np.abs(s - s_ref).max() < tol, prec
+    # the ends exactly: theta = 0 at x = 1 and pi/3 at x = -1
+    c, s = np.zeros(2), np.zeros(2)
+    m.cp3m_third_angle(0, P(np.array([1.0, -1.0])), 2, c.ctypes.data_as(dp),
+                       s.ctypes.data_as(dp))
+    assert c[0] == 1.0 and s[0] == 0.0
+    assert abs(c[1] - 0.5) < 2e-16 and abs(s[1] - np.sqrt(3) / 2) < 2e-16
+
+
 def test_phase_polynomial():
     """m_cispi_m1(f) = e^{2 pi i f} - 1 on |f| <= 1/2 against numpy's sin:
     a few ulp of |w| <= 2 everywhere, exact zero at f = 0, w = -2 at the half
