@@ -31,8 +31,8 @@
 //      pair only ever multiplies a small e^{i psi} - 1.
 //   3. Y is unitary, so only two columns are carried; the third is
 //      det(Y) conj(y_0 x y_1), with det(Y) = e^{i sum psi} accumulated.
-//   4. cos(t0 -+ 2pi/3) = -cos(t0)/2 +- (sqrt(3)/2) sin(t0): one sincos for
-//      the three roots.
+//   4. cos(t0 -+ 2pi/3) = -cos(t0)/2 +- (sqrt(3)/2) sin(t0): one (cos t0,
+//      sin t0) pair for the three roots.
 // Phases: every layer phase is carried in revolutions, reduced to
 // [-1/2, 1/2] in P::D, and e^{i psi} - 1 is evaluated in P::F from minimax
 // polynomials of sin and cos of the half angle (no quadrant logic).
@@ -108,7 +108,6 @@ struct PolicyMixed {
 // operations by precision.
 CP3_HD double m_sqrt(double x) { return sqrt(x); }
 CP3_HD float m_sqrt(float x) { return sqrtf(x); }
-CP3_HD double m_acos(double x) { return acos(x); }
 CP3_HD float m_acos(float x) { return acosf(x); }
 CP3_HD double m_fabs(double x) { return fabs(x); }
 CP3_HD float m_fabs(float x) { return fabsf(x); }
@@ -122,46 +121,6 @@ CP3_HD float m_rsqrt(float x) { return rsqrtf(x); }
 CP3_HD double m_rsqrt(double x) { return 1.0 / sqrt(x); }
 CP3_HD float m_rsqrt(float x) { return 1.0f / sqrtf(x); }
 #endif
-// sin and cos of an FP64 angle, inline (no call, so nothing is spilled
-// around it): Cody-Waite reduction by pi/2 split in three parts (the first
-// two with 33 significant bits, so k*P1 and k*P2 are exact for |k| < 2^20),
-// then the Taylor series of sin and cos on [-pi/4, pi/4] through x^17 and
-// x^18 (truncation below 1e-19).  Accurate to a few ulp for |x| < 1e6 rad.
-CP3_HD void m_sincos(double x, double* s, double* c) {
-  const double kTwoOverPi = 0.6366197723675814;
-  const double P1 = 1.5707963267341256;
-  const double P2 = 6.077100506303966e-11;
-  const double P3 = 2.0222662487959506e-21;
-  const double k = rint(x * kTwoOverPi);
-  double r = fma(-k, P1, x);
-  r = fma(-k, P2, r);
-  r = fma(-k, P3, r);
-  const double z = r * r;
-  double ps = 2.8114572543455206e-15;
-  ps = fma(ps, z, -7.647163731819816e-13);
-  ps = fma(ps, z, 1.6059043836821613e-10);
-  ps = fma(ps, z, -2.505210838544172e-08);
-  ps = fma(ps, z, 2.7557319223985893e-06);
-  ps = fma(ps, z, -0.0001984126984126984);
-  ps = fma(ps, z, 0.008333333333333333);
-  ps = fma(ps, z, -0.16666666666666666);
-  const double sr = fma(r * z, ps, r);
-  double pc = -1.5619206968586225e-16;
-  pc = fma(pc, z, 4.779477332387385e-14);
-  pc = fma(pc, z, -1.1470745597729725e-11);
-  pc = fma(pc, z, 2.08767569878681e-09);
-  pc = fma(pc, z, -2.755731922398589e-07);
-  pc = fma(pc, z, 2.48015873015873e-05);
-  pc = fma(pc, z, -0.001388888888888889);
-  pc = fma(pc, z, 0.041666666666666664);
-  pc = fma(pc, z, -0.5);
-  const double cr = fma(z, pc, 1.0);
-  const int q = (int)(int64_t)k & 3;
-  const double so = (q & 1) ? cr : sr;
-  const double co = (q & 1) ? sr : cr;
-  *s = (q & 2) ? -so : so;
-  *c = ((q + 1) & 2) ? -co : co;
-}
 #if defined(__HIP_DEVICE_COMPILE__)
 // FP32: reduce in revolutions and use the hardware v_sin_f32 / v_cos_f32.
 CP3_HD void m_sincos(float x, float* s, float* c) {
@@ -327,8 +286,9 @@ CP3_HD void eigvec(const D H[6], const D H2[6], D lk, D lp, D lq, F v[3]) {
 // from a seed of FP32 accuracy.  1 / |u|^4 is taken as (2 - |u|^2)^2, which
 // differs from it by O((|u|^2 - 1)^2) and keeps the convergence quadratic
 // (at u = u* (1 + e) the step leaves u* + O(e^2)), so two steps reach FP64
-// rounding: about 50 FP64 operations instead of an FP64 acos and sincos
-// (about 145 instructions on gfx950, acos with a divergent branch).
+// rounding (within one ulp of the exact values over [-1, 1]).  On gfx950 a
+// slab solve takes 357 VALU instructions this way, against 439 with an FP64
+// acos (about 80, with a divergent branch) and sincos (about 65).
 // 1 - x^2 is formed as (1 - x)(1 + x), accurate to a few ulp up to |x| = 1.
 CP3_HD void m_third_angle_seed(double x, double* c, double* s) {
   const float t = acosf((float)x) * (1.0f / 3.0f);

@@ -175,7 +175,6 @@ inline Cnt<double> m_sqrt(Cnt<double> x) { g_ops.sqrt64 += 1; g_ops.f64 += 1; re
 inline Cnt<float> m_sqrt(Cnt<float> x) { g_ops.sqrt32 += 1; g_ops.f32 += 1; return mk<float>(std::sqrt(x.v)); }
 inline Cnt<double> m_rsqrt(Cnt<double> x) { g_ops.sqrt64 += 1; g_ops.f64 += 2; return mk<double>(1.0 / std::sqrt(x.v)); }
 inline Cnt<float> m_rsqrt(Cnt<float> x) { g_ops.sqrt32 += 1; g_ops.f32 += 2; return mk<float>(1.0f / std::sqrt(x.v)); }
-inline Cnt<double> m_acos(Cnt<double> x) { g_ops.acos64 += 1; return mk<double>(std::acos(x.v)); }
 inline Cnt<float> m_acos(Cnt<float> x) { g_ops.acos32 += 1; return mk<float>(std::acos(x.v)); }
 inline Cnt<double> m_fabs(Cnt<double> x) { return mk<double>(std::fabs(x.v)); }
 inline Cnt<float> m_fabs(Cnt<float> x) { return mk<float>(std::fabs(x.v)); }
@@ -192,11 +191,6 @@ inline void m_cispi_m1(Cnt<T> f, Cnt<T>* wr, Cnt<T>* wi) {
   cp3::m_cispi_m1(f.v, &r, &i);
   wr->v = r;
   wi->v = i;
-}
-inline void m_sincos(Cnt<double> x, Cnt<double>* s, Cnt<double>* c) {
-  g_ops.sincos64 += 1;
-  s->v = std::sin(x.v);
-  c->v = std::cos(x.v);
 }
 inline void m_sincos(Cnt<float> x, Cnt<float>* s, Cnt<float>* c) {
   g_ops.sincos32 += 1;
