@@ -281,6 +281,33 @@ struct cp3_server {
   // batch are neither pending nor finished, so no client call reads or frees
   // them meanwhile.
   void enqueue(Slot& s) {
+    // A batch of one request (the usual case at moderate rates) passes its
+    // job by value to the kernels instead of copying a job table to the
+    // device first: the same per-point arithmetic, so the same bits.  In
+    // table mode the pre-pass stores the descriptor for the propagation
+    // kernel, as the single-grid propagator does.
+    if (s.ajobs.size() + s.bjobs.size() == 1) {
+      if (!s.ajobs.empty()) {
+        const cp3::AtmosJob& a = s.ajobs[0];
+        cp3::AtmosJob* dj = (cp3::AtmosJob*)s.d_ajobs.p;
+        if (s.table) {
+          hip_check(cp3::launch_slab_table_single(s.precision, a, dj, s.stream),
+                    "slab table launch");
+          hip_check(cp3::launch_atmos_table(s.precision, s.nc, dj, 1, s.au,
+                                            (int)std::min<int64_t>((s.au + 7) / 8 * 8,
+                                                                   0x7ffffff8),
+                                            s.stream),
+                    "atmospheric launch");
+        } else {
+          hip_check(cp3::launch_atmos_single(s.precision, s.nc, s.w, a, s.stream),
+                    "atmospheric launch");
+        }
+      } else {
+        hip_check(cp3::launch_beam_single(s.precision, s.bjobs[0], s.stream), "beam launch");
+      }
+      hip_check(hipEventRecord(s.done, s.stream), "hipEventRecord");
+      return;
+    }
     const size_t abytes = s.ajobs.size() * sizeof(cp3::AtmosJob);
     const size_t bbytes = s.bjobs.size() * sizeof(cp3::BeamJob);
     std::memcpy(s.staging.p, s.ajobs.data(), abytes);

@@ -37,9 +37,13 @@ def test_alone_equals_in_mix(precision, mode):
         p = W.draw_param_sets(1, 4242)[0]
         probe_a = make_job(ATMOS, agrid[250], p, 1)
         probe_b = make_job(BEAM, bgrid[10000], p, 0)
-        ra, rb = srv.submit(probe_a), srv.submit(probe_b)
+        # alone: each probe is a batch of its own (its job passed by value),
+        # while in the mix below it shares a batch's job table
+        ra = srv.submit(probe_a)
         srv.wait(ra)
+        rb = srv.submit(probe_b)
         srv.wait(rb)
+        assert srv.stats()["batches"] == 2
         alone_a, alone_b = srv.fetch(ra), srv.fetch(rb)
         mix = draw_requests(40, 77, agrid, bgrid)
         ids = [srv.submit(j) for j in mix[:20]]
