@@ -92,7 +92,9 @@ def time_steps(b, steps, warmup, graph):
 def other_configs():
     """Kernel time of the other throughput configurations, median of 5 runs
     after one warm-up: the single-grid propagator on the 12-shell model at
-    2000 x 2000 (0.1-100 GeV, nominal parameters, neutrinos; FP64, mixed, FP32),
+    2000 x 2000 (0.1-100 GeV, nominal parameters, neutrinos; FP64, mixed, FP32;
+    also end to end: calculateProbabilities plus the first getProbability,
+    which copies the whole result to the host),
     the same grid with production-height averaging over 20 bins (edges 5-50 km,
     equal bin probabilities; FP64, mixed; pre-pass plus averaging kernel), and
     the beam batch of 4096 seeded sets x 1e5 energies (0.1-20 GeV, 1300 km,
@@ -119,7 +121,21 @@ def other_configs():
                 p.calculateProbabilities(0)
                 ms.append(p.lastKernelMs())
             t = float(np.median(ms[1:]))
-            single[prec] = {"kernel_ms": t, "evals_per_s": n * n / (t / 1e3)}
+            # the reference's own semantics, end to end: calculateProbabilities
+            # (synchronous) and the first getProbability, which copies the whole
+            # [9][nC][nE] result to pinned host memory over PCIe
+            e2e = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                p.calculateProbabilities(0)
+                p.getProbability(0, 0, 0)
+                e2e.append((time.perf_counter() - t0) * 1e3)
+            te = float(np.median(e2e))
+            out_bytes = 9 * n * n * (8 if prec == "fp64" else 4)
+            single[prec] = {"kernel_ms": t, "evals_per_s": n * n / (t / 1e3),
+                            "end_to_end_ms": te, "end_to_end_evals_per_s": n * n / (te / 1e3),
+                            "result_bytes": out_bytes,
+                            "host_copy_GBps": out_bytes / ((te - t) / 1e3) / 1e9}
     out["single_grid_12shell_2000x2000"] = single
     nb = 20
     prob = np.full(2 * 3 * n * n * nb, 1.0 / nb)
