@@ -250,7 +250,7 @@ __device__ __forceinline__ void atmos_block(const GridDev& g, int sb, int e, boo
   const int lane = (int)threadIdx.x % kWave;
   const int wave = W == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
   const D E = (D)gload(g.ev, e_ok ? e : g.nE - 1);
-  const D inv = D(1) / E;
+  const D inv = m_recip(E);
 
   const int base = sb * (W * NC);
   CosSet<NC> cs;
@@ -535,7 +535,7 @@ __device__ __forceinline__ void slab_wave(const AtmosJob& job, int64_t w) {
   // g.S >= 1 (an Earth model has rows), so row 0 of rye exists
   const D rye = (D)gload(g.rye, i < g.S ? i : 0);
   Slab<P> sl;
-  solve_slab<P>(job.sc, E, D(1) / E, i < g.S ? rye : D(0), sl);
+  solve_slab<P>(job.sc, E, m_recip(E), i < g.S ? rye : D(0), sl);
   TileIO<Slab<P>>::store(const_cast<char*>(slab_tile<P>(job.slabs, g, i, e)), e % kWave, sl);
 }
 

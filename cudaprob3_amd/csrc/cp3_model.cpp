@@ -25,7 +25,7 @@ void atmos_point(const cp3::SetConst& sc, const cp3::TrajTable& tt, int c,
   using D = typename P::D;
   using F = typename P::F;
   const D e = (D)E;
-  const D inv = D(1) / e;
+  const D inv = cp3::m_recip(e);
   const int M = tt.M[(size_t)c];
   cp3::Traj<P> t;
   cp3::traj_init<P>(t, M == 0);
@@ -68,7 +68,7 @@ void atmos_point_avg(const cp3::SetConst& sc, const cp3::TrajTable& tt, int c,
   using D = typename P::D;
   using F = typename P::F;
   const D e = (D)E;
-  const D inv = D(1) / e;
+  const D inv = cp3::m_recip(e);
   const int M = tt.M[(size_t)c];
   cp3::Traj<P> t;
   cp3::traj_init<P>(t, M == 0);
@@ -175,6 +175,8 @@ inline Cnt<double> m_sqrt(Cnt<double> x) { g_ops.sqrt64 += 1; g_ops.f64 += 1; re
 inline Cnt<float> m_sqrt(Cnt<float> x) { g_ops.sqrt32 += 1; g_ops.f32 += 1; return mk<float>(std::sqrt(x.v)); }
 inline Cnt<double> m_rsqrt(Cnt<double> x) { g_ops.sqrt64 += 1; g_ops.f64 += 2; return mk<double>(1.0 / std::sqrt(x.v)); }
 inline Cnt<float> m_rsqrt(Cnt<float> x) { g_ops.sqrt32 += 1; g_ops.f32 += 2; return mk<float>(1.0f / std::sqrt(x.v)); }
+inline Cnt<double> m_recip(Cnt<double> x) { g_ops.f64 += 1; return mk<double>(1.0 / x.v); }
+inline Cnt<float> m_recip(Cnt<float> x) { g_ops.f32 += 1; return mk<float>(1.0f / x.v); }
 inline Cnt<float> m_acos(Cnt<float> x) { g_ops.acos32 += 1; return mk<float>(std::acos(x.v)); }
 inline Cnt<double> m_fabs(Cnt<double> x) { return mk<double>(std::fabs(x.v)); }
 inline Cnt<float> m_fabs(Cnt<float> x) { return mk<float>(std::fabs(x.v)); }
