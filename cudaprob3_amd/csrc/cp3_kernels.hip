@@ -514,43 +514,74 @@ __device__ __forceinline__ const char* slab_tile(const void* slabs, const GridDe
 
 // Pre-pass: the (S + 1) x nE used entries of each job's table, 64
 // consecutive (row, energy) pairs per wave, row-major, so that no lane solves
-// a padding column.  Wave w of job `job`:
-template <class P>
-__device__ __forceinline__ void slab_wave(const AtmosJob& job, int64_t w) {
+// a padding column.  K consecutive such waves w0 .. w0 + K - 1 of job `job`
+// in one wave: every tile's indices and loads first, so that the later
+// tiles' loads are in flight during the first solve, and the set constants
+// copied once (the tile stores would otherwise make the compiler reload them
+// from the job for every tile); then the solves and stores in tile order.
+// A lane past the job's last wave or last row solves a valid pair and does
+// not store.
+template <class P, int K>
+__device__ __forceinline__ void slab_tiles(const AtmosJob& job, int64_t w0) {
   using D = typename P::D;
   const int lane = (int)threadIdx.x;
   const GridDev& g = job.g;
-  if (w >= slab_waves(g)) return;
-  // the wave's first pair (row i0, energy e0), wave-uniform
-  const int64_t first = w * kWave;
-  const int i0 = (int)(first / g.nE);
-  int i = i0;
-  int e = (int)(first - (int64_t)i0 * g.nE) + lane;
-  while (e >= g.nE) {  // at most ceil(64 / nE) steps
-    e -= g.nE;
-    ++i;
+  const int nE = g.nE, S = g.S;
+  const int64_t nw = slab_waves(g);
+  char* const slabs = (char*)job.slabs;
+  const int64_t tiles = energy_tiles(g);
+  int ii[K], ee[K];
+  bool ok[K];
+  D E[K], rye[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t w = w0 + k;
+    // the wave's first pair (row i0, energy e0), wave-uniform
+    const int64_t first = (w < nw ? w : 0) * kWave;
+    const int i0 = (int)(first / nE);
+    int i = i0;
+    int e = (int)(first - (int64_t)i0 * nE) + lane;
+    while (e >= nE) {  // at most ceil(64 / nE) steps
+      e -= nE;
+      ++i;
+    }
+    ok[k] = w < nw && i <= S;
+    ii[k] = i;
+    ee[k] = e;
+    E[k] = (D)gload(g.ev, e);
+    // S >= 1 (an Earth model has rows), so row 0 of rye exists
+    rye[k] = (D)gload(g.rye, i < S ? i : 0);
   }
-  if (i > g.S) return;  // past the last row
-  const D E = (D)gload(g.ev, e);
-  // g.S >= 1 (an Earth model has rows), so row 0 of rye exists
-  const D rye = (D)gload(g.rye, i < g.S ? i : 0);
-  Slab<P> sl;
-  solve_slab<P>(job.sc, E, m_recip(E), i < g.S ? rye : D(0), sl);
-  TileIO<Slab<P>>::store(const_cast<char*>(slab_tile<P>(job.slabs, g, i, e)), e % kWave, sl);
+  const SetConst sc = job.sc;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    Slab<P> sl;
+    solve_slab<P>(sc, E[k], m_recip(E[k]), ii[k] < S ? rye[k] : D(0), sl);
+    // loads and stores share one counter (vmcnt), so a wait for the later
+    // tiles' loads placed after this tile's stores would also wait for the
+    // stores; the loads were issued before this solve and have arrived, so
+    // wait for them here (vmcnt(0), expcnt and lgkmcnt left alone)
+    if (k == 0 && K > 1) __builtin_amdgcn_s_waitcnt(0x0F70);
+    if (ok[k])
+      TileIO<Slab<P>>::store(
+          slabs + ((int64_t)ii[k] * tiles + ee[k] / kWave) * TileIO<Slab<P>>::kTileBytes,
+          ee[k] % kWave, sl);
+  }
 }
 
 // The batch's pre-pass on a 2-D grid (y: job, x: groups of kSlabTiles of the
 // job's waves) so that a workgroup finds its job without a search; tiles past
 // a smaller job's count are skipped at once.  A one-tile wave spends most of
-// its life waiting (its job's loads, its stores); two tiles per wave take the
-// 1024-set, 59-shell batch's pre-pass from 0.485 to 0.448 ms on MI355X (four:
-// 0.46 ms).
-constexpr int kSlabTiles = 2;
+// its life waiting (its job's loads, its stores).  With every tile's loads
+// issued before the first solve, three tiles per wave take the 1024-set,
+// 59-shell batch's pre-pass to 0.414 ms on MI355X, against 0.441 ms with two
+// and 0.417 ms with four (one tile at a time, loads after the previous
+// tile's stores: 0.485 ms with one tile, 0.448 with two).
+constexpr int kSlabTiles = 3;
 template <class P>
 __global__ void __launch_bounds__(kWave) slab_table_kernel(const AtmosJob* __restrict__ jobs) {
   const AtmosJob& job = jobs[blockIdx.y];
-#pragma unroll
-  for (int k = 0; k < kSlabTiles; ++k) slab_wave<P>(job, (int64_t)blockIdx.x * kSlabTiles + k);
+  slab_tiles<P, kSlabTiles>(job, (int64_t)blockIdx.x * kSlabTiles);
 }
 
 // The single grid's pre-pass, its job passed as a kernel argument; workgroup
@@ -566,7 +597,7 @@ __global__ void __launch_bounds__(kWave) slab_table_single_kernel(AtmosJob job,
     uint32_t* out = reinterpret_cast<uint32_t*>(dst);
     for (int i = (int)threadIdx.x; i < kWords; i += kWave) out[i] = src[i];
   }
-  slab_wave<P>(job, (int64_t)blockIdx.x);
+  slab_tiles<P, 1>(job, (int64_t)blockIdx.x);
 }
 
 // The NC trajectories of cosine block sb (NC cosines) for one energy per
