@@ -8,7 +8,10 @@ from a hipGraph).
   calculate_atmos, physics.hpp:789-1150) within 1e-5 on every point;
 * FP64 form of the same batch: sampled jobs within 1e-10 of the oracle;
 * graph replays equal the eager run bitwise, and sampled jobs equal the
-  single-grid propagator bitwise.
+  single-grid propagator bitwise;
+* every point of every job, in both forms: finite, in [0, 1], and each row
+  and column of its 3 x 3 probability matrix summing to 1 (1e-5 mixed,
+  1e-12 FP64).
 
 Every job's last nE mod 64 energies (16 on the 400-energy grid, 8 on the
 200-energy grid) run in packed tail tiles of 4 and 8 jobs; the sampled jobs
@@ -19,7 +22,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import osc_for
+from conftest import conservation_all_jobs, osc_for
 from cudaprob3_amd import workloads as W
 from cudaprob3_amd.batch import ATMOS, Batch, make_job
 
@@ -136,6 +139,9 @@ def test_c3_full_size_mixed_and_fp64(oracle):
             single = np.transpose(p.probabilities(), (2, 0, 1))
         assert np.array_equal(single, mixed.result(j)), f"job {j}"
     keep = {j: mixed.result(j) for j in picks[::4]}
+    # all 2.048e8 points: probability conserved
+    dev = conservation_all_jobs(mixed, len(wl.jobs), TOL_MIXED)
+    print(f"mixed, all {len(wl.jobs)} jobs: sums within {dev:.3e} of 1", flush=True)
     mixed.close()
 
     # the FP64 form of the same batch
@@ -153,6 +159,8 @@ def test_c3_full_size_mixed_and_fp64(oracle):
         worst_mix64 = max(worst_mix64, float(np.abs(keep[j] - got).max()))
         P = got.reshape(3, 3, *got.shape[1:])
         assert max(np.abs(P.sum(0) - 1).max(), np.abs(P.sum(1) - 1).max()) < 1e-12
+    dev64 = conservation_all_jobs(f64, len(wl.jobs), 1e-12)
+    print(f"fp64, all {len(wl.jobs)} jobs: sums within {dev64:.3e} of 1", flush=True)
     f64.close()
     print(f"fp64 vs oracle over {len(fp64_picks)} jobs: max {worst64:.3e}; "
           f"mixed vs fp64 (GPU) max {worst_mix64:.3e}", flush=True)

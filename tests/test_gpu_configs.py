@@ -14,7 +14,7 @@ import time
 import numpy as np
 import pytest
 
-from conftest import osc_for
+from conftest import conservation_all_jobs, osc_for
 from cudaprob3_amd import workloads as W
 from cudaprob3_amd.batch import BEAM, Batch, make_job
 
@@ -82,9 +82,11 @@ def test_beam_batch_4096x1e5_one_launch(oracle):
             worst = max(worst, float(d.max()))
             sq += float((d ** 2).sum())
             n += d.size
+        # all 4.096e8 points: probability conserved
+        dev = conservation_all_jobs(b, 4096, 1e-5)
     print(f"beam 4096 sets x 1e5 E, one mixed launch: {np.median(ms):.3f} ms, "
           f"{rate:.3e} evals/s; sampled 16 sets: max {worst:.2e} "
-          f"rms {np.sqrt(sq / n):.2e}")
+          f"rms {np.sqrt(sq / n):.2e}; all sets: sums within {dev:.2e} of 1")
     assert worst < 1e-5
 
 
