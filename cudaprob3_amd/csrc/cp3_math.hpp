@@ -373,9 +373,13 @@ CP3_HD void matter_eigs(const SetConst& sc, D fac, D lam[3]) {
   D tmp = alpha * alpha - D(3) * beta;
   tmp = tmp < D(0) ? D(0) : tmp;
   const D st = m_sqrt(tmp);
-  D arg = (D(2) * alpha * alpha * alpha - D(9) * alpha * beta +
-           D(27) * gamma) /
-          (D(2) * (tmp * st));
+  // the quotient by a reciprocal and a product (no IEEE division sequence);
+  // a denominator clamped at the smallest normal keeps a triple root (tmp
+  // rounded to 0) finite: a nonzero numerator then clamps to +-1 below, as
+  // its quotient by 0 would, and a zero one gives 0
+  const D den = D(2) * (tmp * st);
+  D arg = (D(2) * alpha * alpha * alpha - D(9) * alpha * beta + D(27) * gamma) *
+          m_recip(den > D(2.2250738585072014e-308) ? den : D(2.2250738585072014e-308));
   if (m_fabs(arg) > D(1)) arg = arg > D(0) ? D(1) : D(-1);
   D sn, cs;
   third_angle(arg, &cs, &sn);
