@@ -33,8 +33,12 @@ class DevBuf:
         self.nbytes = nbytes
         self.p = C.c_void_p()
         assert hip().hipMalloc(C.byref(self.p), C.c_size_t(nbytes)) == 0
-        # a fill the kernels must overwrite everywhere
+        # a fill the kernels must overwrite everywhere; hipMemset returns
+        # before the fill is done and runs on the null stream, which the
+        # library's non-blocking streams do not wait for, so the fill is
+        # completed here before any kernel writes the buffer
         assert hip().hipMemset(self.p, 0x7f, C.c_size_t(nbytes)) == 0
+        assert hip().hipDeviceSynchronize() == 0
 
     def get(self, dtype, shape):
         out = np.empty(shape, dtype=dtype)
