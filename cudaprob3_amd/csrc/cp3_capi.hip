@@ -168,7 +168,9 @@ struct cp3_atmos {
   DevBuf d_ev, d_perm, d_M, d_Lvac, d_seg, d_rye, d_out;
   PinnedBuf h_out;
   int S = 0, rye_cs = 0;
-  bool table_dirty = true, energies_dirty = true, results_on_host = false;
+  bool table_dirty = true, energies_dirty = true;
+  // bit t: channel t of the last calculation is in h_out
+  uint32_t on_host = 0;
   // production-height averaging (atmoscpupropagator.hpp:88-108, 395-427,
   // 532-535): the flag, the bin count, the bin edges (km) and, on the device,
   // the reordered bin probabilities of both neutrino types and the per-cosine
@@ -276,7 +278,7 @@ struct cp3_atmos {
       slabs = d_slabs.p;
       dj = (cp3::AtmosJob*)d_job.p;
     }
-    results_on_host = false;
+    on_host = 0;
     enqueue(nutype, d_out.p, slabs, dj, true);
     launched = true;
   }
@@ -368,13 +370,20 @@ struct cp3_atmos {
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
 
-  void fetch() {
-    if (results_on_host) return;
+  // Channel t ([nC][nE], contiguous in the [9][nC][nE] result) to pinned
+  // host memory on its first access after a calculation.  The reference
+  // copies all nine channels on the first access to any
+  // (atmoscudapropagator.cu:222-235); copying by channel gives the same
+  // values and moves a ninth of the bytes per channel read.
+  void fetch(int t) {
+    if (on_host & (1u << t)) return;
     DeviceScope ds(dev);
-    hip_check(hipMemcpyAsync(h_out.p, d_out.p, out_bytes(), hipMemcpyDeviceToHost, stream),
+    const size_t cb = out_bytes() / 9;
+    hip_check(hipMemcpyAsync((char*)h_out.p + (size_t)t * cb, (const char*)d_out.p + (size_t)t * cb,
+                             cb, hipMemcpyDeviceToHost, stream),
               "result copy");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    results_on_host = true;
+    on_host |= 1u << t;
   }
 
   double value(size_t idx) const {
@@ -393,7 +402,9 @@ struct cp3_beam {
   OscArgs osc;
   DevBuf d_ev, d_out;
   PinnedBuf h_out;
-  bool energies_dirty = true, results_on_host = false;
+  bool energies_dirty = true;
+  // bit t: channel t of the last calculation is in h_out
+  uint32_t on_host = 0;
 
   ~cp3_beam() {
     DeviceScope ds(dev);
@@ -419,7 +430,7 @@ struct cp3_beam {
     job.rye = rho * cp3::kBeamYe;
     job.L = L;
     job.out = d_out.p;
-    results_on_host = false;
+    on_host = 0;
     hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
     hip_check(cp3::launch_beam_single(precision, job, stream), "beam kernel launch");
     hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
@@ -429,13 +440,17 @@ struct cp3_beam {
     DeviceScope ds(dev);
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
   }
-  void fetch() {
-    if (results_on_host) return;
+  // Channel t ([nE]) to pinned host memory on its first access after a
+  // calculation (the reference copies all nine, beamcudapropagator.cu:146-158).
+  void fetch(int t) {
+    if (on_host & (1u << t)) return;
     DeviceScope ds(dev);
-    hip_check(hipMemcpyAsync(h_out.p, d_out.p, out_bytes(), hipMemcpyDeviceToHost, stream),
+    const size_t cb = out_bytes() / 9;
+    hip_check(hipMemcpyAsync((char*)h_out.p + (size_t)t * cb, (const char*)d_out.p + (size_t)t * cb,
+                             cb, hipMemcpyDeviceToHost, stream),
               "result copy");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    results_on_host = true;
+    on_host |= 1u << t;
   }
   double value(size_t idx) const {
     if (precision == CP3_FP64) return ((const double*)h_out.p)[idx];
@@ -746,7 +761,7 @@ int cp3_atmos_get_probability(cp3_atmos* p, int ic, int ie, int t, double* out) 
       throw Error(CP3_ERR_RANGE, "CudaPropagatorSingle::getProbability. Invalid indices");
     check_channel(t);
     if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
-    p->fetch();
+    p->fetch(t);
     *out = p->value(((size_t)t * p->nC + (size_t)ic) * p->nE + (size_t)ie);
   });
 }
@@ -757,7 +772,7 @@ int cp3_atmos_get_probability_arr(cp3_atmos* p, double* out, int t) {
     need_ptr(out, "out");
     check_channel(t);
     if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
-    p->fetch();
+    p->fetch(t);
     const size_t base = (size_t)t * p->nC * p->nE;
     size_t k = 0;
     for (int ie = 0; ie < p->nE; ++ie)
@@ -910,7 +925,7 @@ int cp3_beam_get_probability(cp3_beam* p, int ie, int t, double* out) {
       throw Error(CP3_ERR_RANGE, "CudaPropagatorSingle::getProbability. Invalid indices");
     check_channel(t);
     if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
-    p->fetch();
+    p->fetch(t);
     *out = p->value((size_t)t * p->nE + (size_t)ie);
   });
 }
@@ -921,7 +936,7 @@ int cp3_beam_get_probability_arr(cp3_beam* p, double* out, int t) {
     need_ptr(out, "out");
     check_channel(t);
     if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
-    p->fetch();
+    p->fetch(t);
     for (int ie = 0; ie < p->nE; ++ie) out[ie] = p->value((size_t)t * p->nE + (size_t)ie);
   });
 }

@@ -168,7 +168,9 @@ int cp3_atmos_wait(cp3_atmos* p);
 int cp3_atmos_calculate_async_static(cp3_atmos* p, int nutype, void* d_out);
 
 /* getProbability(ic, ie, t) (atmoscudapropagator.cu:222-235): the first
- * access after a calculate copies the whole result to pinned host memory. */
+ * access to channel t after a calculate copies that channel ([nC][nE]) to
+ * pinned host memory (the reference copies all nine at once; the values are
+ * the same). */
 int cp3_atmos_get_probability(cp3_atmos* p, int index_cosine,
                               int index_energy, int t, double* out);
 /* getProbabilityArr (atmoscudapropagator.cu:238-257): n_energies*n_cosines
@@ -210,7 +212,8 @@ int cp3_beam_set_path_length(cp3_beam* p, double length_km);
 int cp3_beam_calculate(cp3_beam* p, int nutype);
 int cp3_beam_calculate_async(cp3_beam* p, int nutype);
 int cp3_beam_wait(cp3_beam* p);
-/* getProbability(ie, t) (beamcudapropagator.cu:146-159). */
+/* getProbability(ie, t) (beamcudapropagator.cu:146-159): the first access
+ * to channel t after a calculate copies that channel to pinned host memory. */
 int cp3_beam_get_probability(cp3_beam* p, int index_energy, int t,
                              double* out);
 /* getProbabilityArr (beamcudapropagator.cu:162-178). */

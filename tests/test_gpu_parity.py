@@ -544,3 +544,49 @@ def test_calculate_without_model_refused():
             p.calculateProbabilities(0)
             ref = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0)
             assert np.array_equal(p.probabilities(), ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_channel_fetch_follows_recalculation(precision):
+    """A channel reaches the host on its first access after a calculation:
+    reads of single channels in any order, before and after a recalculation
+    with other parameters, equal a fresh propagator's full result bitwise
+    (no stale channel survives the recalculation)."""
+    from cudaprob3_amd import AtmosPropagator, BeamPropagator
+    e = W.earth_12()
+    cz, ev = W.cos_grid(40), W.log_grid(56, 0.5, 50.0)
+    other = list(NOM)
+    other[3], other[5] = 0.7, 2.6e-3
+    ref_a = run_atmos(cz, ev, e.radii, e.rhos, e.yps, NOM, 0, precision)
+    ref_b = run_atmos(cz, ev, e.radii, e.rhos, e.yps, other, 0, precision)
+    assert not np.array_equal(ref_a[:, :, 4], ref_b[:, :, 4])
+    with AtmosPropagator(len(cz), len(ev), precision=precision) as p:
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        p.setDensity(e.radii, e.rhos, e.yps)
+        p.setProductionHeight(22.0)
+        for par, ref in ((NOM, ref_a), (other, ref_b)):
+            t12, t13, t23, d, m12, m23 = par
+            p.setMNSMatrix(t12, t13, t23, d, 1)
+            p.setNeutrinoMasses(m12, m23)
+            p.calculateProbabilities(0)
+            assert p.getProbability(3, 5, 4) == ref[3, 5, 4]
+            arr = p.getProbabilityArr(7).reshape(len(ev), len(cz)).T
+            assert np.array_equal(arr, ref[:, :, 7])
+            assert p.getProbability(39, 55, 0) == ref[39, 55, 0]
+        assert np.array_equal(p.probabilities(), ref_b)
+    bev = W.log_grid(300, 0.1, 20.0)
+    rb_a = run_beam(bev, NOM, 0, precision)
+    rb_b = run_beam(bev, other, 0, precision)
+    with BeamPropagator(len(bev), precision=precision) as p:
+        p.setEnergyList(bev)
+        p.setDensity(2.6)
+        p.setPathLength(1300.0)
+        for par, ref in ((NOM, rb_a), (other, rb_b)):
+            t12, t13, t23, d, m12, m23 = par
+            p.setMNSMatrix(t12, t13, t23, d, 1)
+            p.setNeutrinoMasses(m12, m23)
+            p.calculateProbabilities(0)
+            assert p.getProbability(17, 2) == ref[17, 2]
+            assert np.array_equal(p.getProbabilityArr(8), ref[:, 8])
+        assert np.array_equal(p.probabilities(), rb_b)
