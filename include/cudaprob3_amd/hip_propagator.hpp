@@ -288,10 +288,10 @@ class HipPropagatorSingle : public AtmosPropagator<FLOAT_T> {
   void calculateAtmosProbabilitiesAsync(NeutrinoType type) { calculateProbabilitiesAsync(type); }
   void waitForCompletion() { check(cp3_atmos_wait(live())); }
   // atmoscudapropagator.cu:319-325: waits, then copies the whole result to
-  // host memory (the first probability read does exactly that).
+  // host memory (the first read of a channel copies that channel).
   void getResultFromDevice() {
     double v = 0;
-    check(cp3_atmos_get_probability(live(), 0, 0, 0, &v));
+    for (int t = 0; t < 9; ++t) check(cp3_atmos_get_probability(live(), 0, 0, t, &v));
   }
   // atmoscpupropagator.hpp:431-444: the shell counts are rebuilt with the
   // trajectory table whenever the cosines or the model change.
@@ -404,7 +404,7 @@ class HipBeamPropagatorSingle : public BeamPropagator<FLOAT_T> {
   void waitForCompletion() { check(cp3_beam_wait(live())); }
   void getResultFromDevice() {
     double v = 0;
-    check(cp3_beam_get_probability(live(), 0, 0, &v));
+    for (int t = 0; t < 9; ++t) check(cp3_beam_get_probability(live(), 0, t, &v));
   }
   // beamcudapropagator.cu:146-159
   FLOAT_T getProbability(int index_energy, ProbType t) override {
