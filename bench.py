@@ -4,7 +4,9 @@ one MI355X.
 Workload (default): 1024 seeded parameter sets on the 59-shell seeded Earth
 model, half on 400 x 400 and half on 200 x 200 (cos, E) grids, E 0.5-50 GeV,
 each for neutrinos and antineutrinos: 2048 jobs, 2.048e8 evaluations per step,
-in one persistent launch replayed from a hipGraph.  A step is one pass over
+in one hipGraph replay of the batch's launches (slab-table pre-pass,
+propagation of the jobs' own energy tiles, packed tails; one launch each over
+all 2048 jobs).  A step is one pass over
 the whole batch with every input resident in HBM.  The mixed form (FP64
 eigen-solve and phases, FP32 amplitudes) is the headline; the plain FP64 form
 of the same batch is timed beside it.
@@ -439,8 +441,9 @@ def main():
         "config": {
             "workload": f"atmos: {args.sets} param sets x (nu, nubar), 59-shell "
                         "seeded Earth, half 400x400 half 200x200 (cos x E), "
-                        "E 0.5-50 GeV, h 22 km; one persistent "
-                        + ("hipGraph-replayed " if graph else "") + "launch",
+                        "E 0.5-50 GeV, h 22 km; one launch per phase over all "
+                        "jobs (pre-pass, propagation, packed tails)"
+                        + (", replayed from a hipGraph" if graph else ""),
             "jobs": len(wl.jobs), "evals_per_step": evals,
             "precision": "mixed",
             "tuning": {"cosine_block": tune_m[0], "group_waves": tune_m[1],

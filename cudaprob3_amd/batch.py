@@ -1,6 +1,7 @@
 """Batched evaluation of many parameter sets on resident grids (the
-``cp3_batch_*`` part of the C ABI): one persistent launch per kind over all
-jobs, optional hipGraph replay, autotuned cosine block and grid size."""
+``cp3_batch_*`` part of the C ABI): one launch per kind and phase over all
+jobs (persistent or one workgroup per work unit, as the autotuner finds
+faster), optional hipGraph replay, autotuned cosine block and grid size."""
 from __future__ import annotations
 
 import ctypes as C
