@@ -321,9 +321,9 @@ CP3_HD void eigvec(const D H[6], const D H2[6], D lk, D lp, D lq, F v[3]) {
 // from a seed of FP32 accuracy.  1 / |u|^4 is taken as (2 - |u|^2)^2, which
 // differs from it by O((|u|^2 - 1)^2) and keeps the convergence quadratic
 // (at u = u* (1 + e) the step leaves u* + O(e^2)), so two steps reach FP64
-// rounding (within one ulp of the exact values over [-1, 1]).  On gfx950 a
-// slab solve takes 357 VALU instructions this way, against 439 with an FP64
-// acos (about 80, with a divergent branch) and sincos (about 65).
+// rounding (within one ulp of the exact values over [-1, 1]).  On gfx950
+// this took a slab solve from 439 VALU instructions, with an FP64 acos
+// (about 80, with a divergent branch) and sincos (about 65), to 357.
 // 1 - x^2 is formed as (1 - x)(1 + x), accurate to a few ulp up to |x| = 1.
 CP3_HD void m_third_angle_seed(double x, double* c, double* s) {
   const float t = acosf((float)x) * (1.0f / 3.0f);
