@@ -18,7 +18,6 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
-import threading
 import time
 
 import numpy as np
@@ -156,19 +155,15 @@ def draw_requests(n, seed, agrid, bgrid):
 def run_stream(precision="mixed", rate=2000.0, duration_s=3.0, warmup_s=0.5,
                seed=2000, slots=4, mem_cap_bytes=32 << 30):
     """Poisson arrivals; returns latency percentiles (microseconds, submit ->
-    result resident in HBM) and throughput."""
-    import sys
-    old_switch = sys.getswitchinterval()
-    sys.setswitchinterval(1e-4)  # the submitter spins; keep the collector responsive
-    try:
-        return _run_stream(precision, rate, duration_s, warmup_s, seed, slots,
-                           mem_cap_bytes)
-    finally:
-        sys.setswitchinterval(old_switch)
+    result resident in HBM, timed inside the server) and throughput.
 
-
-def _run_stream(precision, rate, duration_s, warmup_s, seed, slots,
-                mem_cap_bytes):
+    One thread submits each request at its arrival time and, between
+    arrivals, polls the oldest outstanding request without blocking and
+    releases it once done.  (A separate collector thread would have to take
+    the GIL from the spinning submitter for every call; above about 5000
+    requests/s it then released results more slowly than they arrived, and
+    the unreleased ones filled the memory cap and stalled admission.)"""
+    from collections import deque
     with Server(precision, mem_cap_bytes, slots) as srv:
         agrid, bgrid = serving_setup(srv)
         n = int(rate * (duration_s + warmup_s)) + 1
@@ -176,33 +171,40 @@ def _run_stream(precision, rate, duration_s, warmup_s, seed, slots,
         rng = np.random.default_rng(seed + 7)
         arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
         ids = [None] * n
-        t_sub = np.zeros(n)
-        done = threading.Event()
-
-        def submitter():
-            t0 = time.perf_counter()
-            for i in range(n):
-                while time.perf_counter() - t0 < arrivals[i]:
-                    pass
-                t_sub[i] = time.perf_counter() - t0
-                ids[i] = srv.submit(jobs[i])
-            done.set()
-
-        th = threading.Thread(target=submitter)
-        th.start()
+        lib, h = srv._lib, srv._h
         lat, queue, evals = [], [], 0
-        for i in range(n):
-            while ids[i] is None:
-                time.sleep(20e-6)
-            srv.wait(ids[i])
-            q, t = srv.latency(ids[i])
-            if arrivals[i] >= warmup_s:
-                lat.append(t)
-                queue.append(q)
-                shape = srv.req_shape[ids[i]]
+        outstanding = deque()
+        q, t = C.c_double(), C.c_double()
+        t0 = time.perf_counter()
+        i = 0
+        while i < n or outstanding:
+            if i < n and time.perf_counter() - t0 >= arrivals[i]:
+                ids[i] = srv.submit(jobs[i])
+                outstanding.append(i)
+                i += 1
+                continue
+            if not outstanding:
+                continue
+            k = outstanding[0]
+            rc = lib.cp3_server_wait(h, C.c_int64(ids[k]), 0)
+            if rc == N.CP3_ERR_STATE:
+                # not finished yet: each poll takes the server's lock, which
+                # its scheduler needs too, so the next one comes 5 us later
+                # (or at the next arrival)
+                t_next = time.perf_counter() + 5e-6
+                while time.perf_counter() < t_next and not (
+                        i < n and time.perf_counter() - t0 >= arrivals[i]):
+                    pass
+                continue
+            N.check(rc)
+            N.check(lib.cp3_server_latency(h, C.c_int64(ids[k]), C.byref(q), C.byref(t)))
+            if arrivals[k] >= warmup_s:
+                lat.append(t.value)
+                queue.append(q.value)
+                shape = srv.req_shape[ids[k]]
                 evals += int(np.prod(shape[1:]))
-            srv.release(ids[i])
-        th.join()
+            srv.release(ids[k])
+            outstanding.popleft()
         st = srv.stats()
         measured_span = arrivals[-1] - warmup_s
     lat, queue = np.array(lat), np.array(queue)
