@@ -98,7 +98,9 @@ def other_configs():
     also end to end: calculateProbabilities plus a first access to every
     channel, which copies the whole result to the host, and plus a first
     access to one channel, which copies that channel),
-    the same grid with production-height averaging over 20 bins (edges 5-50 km,
+    the reference's example size (4 shells, 100 x 100; FP64 and mixed; kernel
+    time and the wall time of a synchronous calculateProbabilities),
+    the same 12-shell grid with production-height averaging over 20 bins (edges 5-50 km,
     equal bin probabilities; FP64, mixed; pre-pass plus averaging kernel), and
     the beam batch of 4096 seeded sets x 1e5 energies (0.1-20 GeV, 1300 km,
     2.6 g/cm^3) in one launch (mixed, the configuration's form, then FP32 and
@@ -153,6 +155,31 @@ def other_configs():
                             "result_bytes": out_bytes,
                             "host_copy_GBps": out_bytes / ((te - t) / 1e3) / 1e9}
     out["single_grid_12shell_2000x2000"] = single
+    # the reference's own example size (4-shell model, 100 x 100, c0): a
+    # calculation this small is bound by launch latency, so its wall time
+    # (calculateProbabilities, synchronous) is reported beside the kernel's
+    e4 = W.prem4()
+    c0 = {}
+    cz0, ev0 = W.cos_grid(100), W.log_grid(100, 0.1, 100.0)
+    for prec in ("fp64", "mixed"):
+        with AtmosPropagator(100, 100, precision=prec) as p:
+            p.setEnergyList(ev0)
+            p.setCosineList(cz0)
+            p.setMNSMatrix(t12, t13, t23, d, k)
+            p.setNeutrinoMasses(m12, m23)
+            p.setDensity(e4.radii, e4.rhos, e4.yps)
+            p.setProductionHeight(W.PROD_HEIGHT_KM)
+            ms, wall = [], []
+            for _ in range(60):
+                t0 = time.perf_counter()
+                p.calculateProbabilities(0)
+                wall.append((time.perf_counter() - t0) * 1e3)
+                ms.append(p.lastKernelMs())
+            t = float(np.median(ms[10:]))
+            tw = float(np.median(wall[10:]))
+            c0[prec] = {"kernel_ms": t, "calculate_wall_ms": tw,
+                        "evals_per_s_wall": 1e4 / (tw / 1e3)}
+    out["c0_4shell_100x100"] = c0
     nb = 20
     prob = np.full(2 * 3 * n * n * nb, 1.0 / nb)
     edges = np.linspace(5.0, 50.0, nb + 1)
