@@ -100,6 +100,8 @@ def other_configs():
     access to one channel, which copies that channel),
     the reference's example size (4 shells, 100 x 100; FP64 and mixed; kernel
     time and the wall time of a synchronous calculateProbabilities),
+    the quadratic-density 4-shell model on the 12-shell grid's cosines and
+    energies (polynomial path; FP64, mixed),
     the same 12-shell grid with production-height averaging over 20 bins (edges 5-50 km,
     equal bin probabilities; FP64, mixed; pre-pass plus averaging kernel), and
     the beam batch of 4096 seeded sets x 1e5 energies (0.1-20 GeV, 1300 km,
@@ -180,6 +182,26 @@ def other_configs():
             c0[prec] = {"kernel_ms": t, "calculate_wall_ms": tw,
                         "evals_per_s_wall": 1e4 / (tw / 1e3)}
     out["c0_4shell_100x100"] = c0
+    # the quadratic-density 4-shell model (polynomial path): densities differ
+    # per trajectory, so every (cosine, shell, energy) is its own solve, in
+    # the propagation kernel (one cosine per wave)
+    ep = W.prem4_poly()
+    poly = {}
+    for prec in ("fp64", "mixed"):
+        with AtmosPropagator(n, n, precision=prec) as p:
+            p.setEnergyList(ev)
+            p.setCosineList(cz)
+            p.setMNSMatrix(t12, t13, t23, d, k)
+            p.setNeutrinoMasses(m12, m23)
+            p.setDensity(ep.radii, ep.a, ep.b, ep.c, ep.yps)
+            p.setProductionHeight(W.PROD_HEIGHT_KM)
+            ms = []
+            for _ in range(6):
+                p.calculateProbabilities(0)
+                ms.append(p.lastKernelMs())
+            t = float(np.median(ms[1:]))
+            poly[prec] = {"kernel_ms": t, "evals_per_s": n * n / (t / 1e3)}
+    out["poly_4shell_2000x2000"] = poly
     nb = 20
     prob = np.full(2 * 3 * n * n * nb, 1.0 / nb)
     edges = np.linspace(5.0, 50.0, nb + 1)
@@ -338,6 +360,53 @@ def cpu_baseline(wl, gpu_batch, budget_s=12.0, budget_1t_s=4.0):
         "one_thread": {"value": evals1 / elapsed1, "unit": "evals/s",
                        "sample": f"{used1} jobs ({evals1} evals, {elapsed1:.1f} s)"},
     }, worst, float(np.sqrt(sq / max(npts, 1)))
+
+
+def cpu_baseline_poly(budget_s=4.0):
+    """The reference's polynomial-density path (the quadratic 4-shell model,
+    setDensity(radii, a, b, c, yps); atmoscpupropagator.hpp:181-242) timed on
+    a bounded sample of the 2000 x 2000 grid: chunks of 20 cosines spread over
+    the range, every energy, with the host cores given to it; the GPU's FP64
+    result of the same points is compared to its output."""
+    sys.path.insert(0, ROOT)
+    from cudaprob3_amd import AtmosPropagator
+    from oracle.loader import Osc, load_oracle, load_ref
+    lib = load_ref(deep=True)
+    kind = "reference"
+    if lib is None:
+        lib, kind = load_oracle(), "port"
+    _, cores = host_cores()
+    ep = W.prem4_poly()
+    n = 2000
+    cz, ev = W.cos_grid(n), W.log_grid(n, 0.1, 100.0)
+    t12, t13, t23, d, k, m12, m23 = W.NOMINAL.for_nutype(0)
+    with AtmosPropagator(n, n, precision="fp64") as p:
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        p.setMNSMatrix(t12, t13, t23, d, k)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(ep.radii, ep.a, ep.b, ep.c, ep.yps)
+        p.setProductionHeight(W.PROD_HEIGHT_KM)
+        p.calculateProbabilities(0)
+        gpu = p.probabilities()  # [cos][E][9]
+    osc = Osc(t12, t13, t23, d, m12, m23, k)
+    evals, elapsed, worst, chunks = 0, 0.0, 0.0, 0
+    for c0 in range(0, n, 40):
+        sl = slice(c0, c0 + 20)
+        t0 = time.perf_counter()
+        ref = lib.atmos_poly(0, cz[sl], ev, ep.radii, ep.a, ep.b, ep.c, ep.yps, osc,
+                             W.PROD_HEIGHT_KM, threads=cores)
+        elapsed += time.perf_counter() - t0
+        evals += ref.shape[0] * ref.shape[1]
+        worst = max(worst, float(np.abs(gpu[sl] - ref).max()))
+        chunks += 1
+        if elapsed >= budget_s:
+            break
+    return {"value": evals / elapsed, "unit": "evals/s", "cores": cores, "kind": kind,
+            "sample": f"{chunks} chunks of 20 of 2000 cosines x 2000 energies "
+                      f"({evals} evals, {elapsed:.2f} s, AtmosCpuPropagator<double>, "
+                      "quadratic 4-shell model)",
+            "gpu_fp64_max_abs_diff": worst}
 
 
 def calib_copy(nbytes):
@@ -575,6 +644,7 @@ def main():
 
     if not args.no_cpu:
         cb, worst, rms = cpu_baseline(wl, mixed_results)
+        cb["poly_4shell_2000x2000"] = cpu_baseline_poly()
         result["cpu_baseline"] = cb
         result["mixed_err_vs_cpu_reference"] = {"max": worst, "rms": rms}
         result["gpu_over_cpu"] = value / cb["value"]
