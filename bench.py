@@ -409,6 +409,62 @@ def cpu_baseline_poly(budget_s=4.0):
             "gpu_fp64_max_abs_diff": worst}
 
 
+def cpu_baseline_pha(budget_s=4.0):
+    """The reference's production-height averaging (20 bins, edges 5-50 km,
+    equal bin probabilities; physics.hpp:997-1067) on the 12-shell 2000 x 2000
+    grid, timed on a bounded sample (chunks of 20 cosines spread over the
+    range, every energy) with the host cores given to it; the GPU's FP64
+    result of the same points is compared to its output.  The compiled
+    reference's bin sinc takes a series for every |x| < 1 (its abs()
+    truncates to an integer, math.hpp:50), the GPU sin(x)/x, so the two differ
+    by that series' truncation, not by rounding."""
+    sys.path.insert(0, ROOT)
+    from cudaprob3_amd import AtmosPropagator
+    from oracle.loader import Osc, load_oracle, load_ref
+    lib = load_ref(deep=True)
+    kind = "reference"
+    if lib is None:
+        lib, kind = load_oracle(), "port"
+    _, cores = host_cores()
+    e = W.earth_12()
+    n, nb = 2000, 20
+    cz, ev = W.cos_grid(n), W.log_grid(n, 0.1, 100.0)
+    edges = np.linspace(5.0, 50.0, nb + 1)
+    t12, t13, t23, d, k, m12, m23 = W.NOMINAL.for_nutype(0)
+    with AtmosPropagator(n, n, precision="fp64") as p:
+        p.setEnergyList(ev)
+        p.setCosineList(cz)
+        p.setMNSMatrix(t12, t13, t23, d, k)
+        p.setNeutrinoMasses(m12, m23)
+        p.setDensity(e.radii, e.rhos, e.yps)
+        p.setProductionHeight(W.PROD_HEIGHT_KM)
+        p.SetNumberOfProductionHeightBinsForAveraging(nb)
+        p.setProductionHeightAveraging(True)
+        p.setProductionHeightList(np.full(2 * 3 * n * n * nb, 1.0 / nb), edges)
+        p.calculateProbabilities(0)
+        gpu = p.probabilities()  # [cos][E][9]
+    osc = Osc(t12, t13, t23, d, m12, m23, k)
+    prob = np.full(2 * 3 * n * 20 * nb, 1.0 / nb)
+    evals, elapsed, worst, chunks = 0, 0.0, 0.0, 0
+    for c0 in range(0, n, 40):
+        sl = slice(c0, c0 + 20)
+        t0 = time.perf_counter()
+        ref = lib.atmos_pha(0, cz[sl], ev, e.radii, e.rhos, e.yps, osc, W.PROD_HEIGHT_KM,
+                            prob, edges, threads=cores)
+        elapsed += time.perf_counter() - t0
+        evals += ref.shape[0] * ref.shape[1]
+        worst = max(worst, float(np.abs(gpu[sl] - ref).max()))
+        chunks += 1
+        if elapsed >= budget_s:
+            break
+    return {"value": evals / elapsed, "unit": "evals/s", "cores": cores, "kind": kind,
+            "sample": f"{chunks} chunks of 20 of 2000 cosines x 2000 energies "
+                      f"({evals} evals, {elapsed:.2f} s, AtmosCpuPropagator<double> with "
+                      "production-height averaging over 20 bins)",
+            "gpu_fp64_max_abs_diff": worst,
+            "note": "the compiled reference's bin sinc is a series for |x| < 1, the GPU's sin(x)/x"}
+
+
 def calib_copy(nbytes):
     """One hipMemcpy device-to-device of nbytes (a plain copy kernel of known
     traffic: nbytes read, nbytes written)."""
@@ -645,6 +701,7 @@ def main():
     if not args.no_cpu:
         cb, worst, rms = cpu_baseline(wl, mixed_results)
         cb["poly_4shell_2000x2000"] = cpu_baseline_poly()
+        cb["height_avg_12shell_2000x2000_20bins"] = cpu_baseline_pha()
         result["cpu_baseline"] = cb
         result["mixed_err_vs_cpu_reference"] = {"max": worst, "rms": rms}
         result["gpu_over_cpu"] = value / cb["value"]
