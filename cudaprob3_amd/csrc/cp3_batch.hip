@@ -295,9 +295,10 @@ struct cp3_batch {
     // the table replaces the waves' shared solves: one wave per workgroup
     w = mode == CP3_SOLVE_TABLE ? 1 : (tune_w ? tune_w : 4);
     // Table-mode tails take at most 4 cosines per lane: on the 1024-set
-    // 59-shell batch (MI355X) the tail kernel takes 0.345 ms at block 4 and
-    // 0.40 ms at blocks 8 and 2, while the propagation kernel is fastest at
-    // block 8 or 4 (6.78 and 6.92 ms).
+    // 59-shell batch (MI355X) the tail kernel takes 0.350 ms at block 4 and
+    // 0.360 ms at block 8 (0.40 ms at blocks 8 and 2 before block-4 kernels
+    // finished their trajectories in pairs), while the propagation kernel is
+    // fastest at block 8.
     tnc = mode == CP3_SOLVE_TABLE ? std::min(nc, 4) : nc;
     int64_t u = 0;
     for (size_t j = 0; j < jobs.size(); ++j) {
